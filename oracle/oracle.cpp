@@ -1,0 +1,1736 @@
+// oracle.cpp -- CPU oracle for pcr's render path.
+//
+// TEST INFRASTRUCTURE ONLY: the checker that tests/, the smoke entry and the
+// CPU-baseline leg of bench.py compare the HIP path against.  The product
+// library never links it.  Built with g++ -std=c++20 -O2 -ffp-contract=off on
+// x86-64 (SSE, no FMA) so every + - * / sqrt rounds exactly as the reference's
+// own build does.
+//
+// Parity status of each part (the checks live in tests/test_oracle_*.py):
+//   NumGen streams            pinned: tests/golden/numgen.npz (compiled reference)
+//   CIE / Spectrum / SPD      pinned: tests/golden/spectral.npz + reference unit-test values
+//   sigmoid fit (homotopy)    pinned: tests/golden/spectral.npz (fitSigmoidCoefficients outputs)
+//   optics                    pinned: tests/golden/optics.npz + reference unit-test values
+//   BVH build + traversal     pinned: tests/golden/bvh_*.npz (node arrays, permutations, hits)
+//   branched estimator        pinned: reference seeded golden PNGs, byte-identical
+//   non-branching estimator   pinned: closed forms (furnace), and statistically (3 sigma)
+//                             against the branched estimator at samples=1, which is the
+//                             reference's own non-branching special case
+//   IEEE-only math (fm_*)     pinned: against float64 numpy references (<= 2 ulp)
+
+#include "oracle.h"
+#include "../pcr_amd/host/procmesh.h"  // input geometry generator (not under test)
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <numbers>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- vectors ----
+// Plain value semantics.  The reference's mutating `a + b` on a non-const
+// lvalue (code/Includes/Vec3f.h:67-73) is reproduced explicitly where it
+// matters (the `hit` drift in the branched estimator).
+struct V3 {
+    float x = 0.f, y = 0.f, z = 0.f;
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 v3(float a, float b, float c) { V3 r; r.x = a; r.y = b; r.z = c; return r; }
+inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+inline V3 mul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalize(V3 a) {
+    float m = std::sqrt(dot(a, a));
+    return v3(a.x / m, a.y / m, a.z / m);
+}
+inline V3 ld3(const float *p) { return v3(p[0], p[1], p[2]); }
+
+const float kPiF = (float)std::numbers::pi;  // 3.14159274f
+const float kEps3 = (float)1e-3;             // the reference's `N * 1e-3`
+
+// ----------------------------------------------------------------- NumGen ----
+// PCG32 XSH-RR (code/NumGen.cpp:26-35); stream seeding NumGen.cpp:59-71.
+inline uint32_t pcg_step(uint64_t &state) {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + 1442695040888963407ULL;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+inline uint64_t stream_seed(uint64_t seed, uint64_t k) {
+    uint64_t s = seed ^ (k * 6364136223846793005ULL + 1442695040888963407ULL);
+    return s == 0 ? 1ULL : s;
+}
+struct Rng {
+    uint64_t state;
+    float next() { return (float)pcg_step(state) * (1.0f / 4294967296.0f); }
+};
+
+// --------------------------------------------------------------- Spectrum ----
+constexpr int kSamples = 61;
+constexpr float kLambdaMin = 400.f, kLambdaMax = 700.f;
+constexpr float kStep = (kLambdaMax - kLambdaMin) / (kSamples - 1);
+inline float lambda_at(int i) { return kLambdaMin + i * kStep; }
+
+// Spectrum::operator() (code/Includes/Spectrum.h:82-90)
+inline float spec_eval(const float *s, float lambda) {
+    if (lambda < kLambdaMin || lambda > kLambdaMax) return 0.f;
+    float t = (lambda - kLambdaMin) / kStep;
+    int i = (int)t;
+    if (i >= kSamples - 1) return s[kSamples - 1];
+    float f = t - i;
+    return s[i] * (1.f - f) + s[i + 1] * f;
+}
+
+// ------------------------------------------------------------------- CIE -----
+// Wyman 2013 fit as in code/Includes/CIE.h:27-53 (glibc expf).
+inline float wyman(float l, float mu, float s1, float s2) {
+    float t = (l < mu) ? (l - mu) * s1 : (l - mu) * s2;
+    return std::exp(-0.5f * t * t);
+}
+inline float xbar(float l) {
+    return 0.362f * wyman(l, 442.0f, 0.0624f, 0.0374f) + 1.056f * wyman(l, 599.8f, 0.0264f, 0.0323f) -
+           0.065f * wyman(l, 501.1f, 0.0490f, 0.0382f);
+}
+inline float ybar(float l) {
+    return 0.821f * wyman(l, 568.8f, 0.0213f, 0.0247f) + 0.286f * wyman(l, 530.9f, 0.0613f, 0.0322f);
+}
+inline float zbar(float l) {
+    return 1.217f * wyman(l, 437.0f, 0.0845f, 0.0278f) + 0.681f * wyman(l, 459.0f, 0.0385f, 0.0725f);
+}
+// CIE.h:79-88
+float ybar_integral() {
+    static const float val = []() {
+        float sum = 0.f;
+        for (int i = 0; i < kSamples; i++) sum += ybar(lambda_at(i));
+        return sum * kStep;
+    }();
+    return val;
+}
+// CIE.cpp:14-28
+V3 spectrum_to_xyz(const float *s) {
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    for (int i = 0; i < kSamples; i++) {
+        float l = lambda_at(i);
+        float v = s[i];
+        X += v * xbar(l);
+        Y += v * ybar(l);
+        Z += v * zbar(l);
+    }
+    float scale = kStep / ybar_integral();
+    return v3(X * scale, Y * scale, Z * scale);
+}
+// CIE.h:97-123
+inline V3 srgb_to_xyz(V3 c) {
+    float R = c.x, G = c.y, B = c.z;
+    return v3(0.4124564f * R + 0.3575761f * G + 0.1804375f * B, 0.2126729f * R + 0.7151522f * G + 0.0721750f * B,
+              0.0193339f * R + 0.1191920f * G + 0.9503041f * B);
+}
+inline V3 xyz_to_srgb(V3 c) {
+    float X = c.x, Y = c.y, Z = c.z;
+    return v3(3.2404542f * X - 1.5371385f * Y - 0.4985314f * Z, -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z,
+              0.0556434f * X - 0.2040259f * Y + 1.0572252f * Z);
+}
+// CIE.h:140-159
+inline V3 single_lambda_xyz(float l, float rad) {
+    float scale = (kLambdaMax - kLambdaMin) / ybar_integral();
+    return v3(rad * xbar(l) * scale, rad * ybar(l) * scale, rad * zbar(l) * scale);
+}
+
+// ---------------------------------------------------- IEEE-only math set ----
+// Used by the non-branching estimator so that a GPU implementation with the
+// same operation order produces identical bits.  Only + - * / floor and
+// correctly rounded sqrt appear; constants are Taylor coefficients.
+inline void fm_sincos_turns(float t, float &s, float &c) {
+    float q = std::floor(t * 4.f + 0.5f);
+    int n = (int)q;
+    float r = t - q * 0.25f;           // exact: |r| <= 1/8
+    float a = r * 6.28318548f;         // float(2*pi)
+    float a2 = a * a;
+    float ps = -1.f / 6.f + a2 * (1.f / 120.f + a2 * (-1.f / 5040.f + a2 * (1.f / 362880.f)));
+    float sn = a + a * (a2 * ps);
+    float pc = -0.5f + a2 * (1.f / 24.f + a2 * (-1.f / 720.f + a2 * (1.f / 40320.f + a2 * (-1.f / 3628800.f))));
+    float cs = 1.f + a2 * pc;
+    switch (n & 3) {
+    case 0: s = sn; c = cs; break;
+    case 1: s = cs; c = -sn; break;
+    case 2: s = -sn; c = -cs; break;
+    default: s = -cs; c = sn; break;
+    }
+}
+inline float fm_exp(float x) {
+    if (x < -103.f) return 0.f;
+    if (x > 88.f) return INFINITY;
+    float kf = std::floor(x * 1.44269504f + 0.5f);
+    float r = x - kf * 0.693145751953125f;
+    r = r - kf * 1.42860677e-06f;
+    float p = 1.f + r * (1.f + r * (0.5f + r * (1.f / 6.f + r * (1.f / 24.f + r * (1.f / 120.f +
+                  r * (1.f / 720.f + r * (1.f / 5040.f)))))));
+    int k = (int)kf;
+    if (k >= -126) {
+        uint32_t b = (uint32_t)(k + 127) << 23;
+        float sc;
+        std::memcpy(&sc, &b, 4);
+        return p * sc;
+    }
+    uint32_t b1 = (uint32_t)(k + 64 + 127) << 23;
+    float s1, s2;
+    std::memcpy(&s1, &b1, 4);
+    uint32_t b2 = (uint32_t)(-64 + 127) << 23;
+    std::memcpy(&s2, &b2, 4);
+    return (p * s1) * s2;
+}
+inline float fm_wyman(float l, float mu, float s1, float s2) {
+    float t = (l < mu) ? (l - mu) * s1 : (l - mu) * s2;
+    return fm_exp(-0.5f * t * t);
+}
+inline V3 fm_cmf(float l) {
+    float xb = 0.362f * fm_wyman(l, 442.0f, 0.0624f, 0.0374f) + 1.056f * fm_wyman(l, 599.8f, 0.0264f, 0.0323f) -
+               0.065f * fm_wyman(l, 501.1f, 0.0490f, 0.0382f);
+    float yb = 0.821f * fm_wyman(l, 568.8f, 0.0213f, 0.0247f) + 0.286f * fm_wyman(l, 530.9f, 0.0613f, 0.0322f);
+    float zb = 1.217f * fm_wyman(l, 437.0f, 0.0845f, 0.0278f) + 0.681f * fm_wyman(l, 459.0f, 0.0385f, 0.0725f);
+    return v3(xb, yb, zb);
+}
+
+// --------------------------------------------------------- RGB->spectrum ----
+// code/Includes/RGBToSpectrum.h:42-63,122-129 and code/RGBToSpectrum.cpp:27-293.
+inline float sigmoid(float x) { return 0.5f + x / (2.f * std::sqrt(1.f + x * x)); }
+inline float sigmoid_d(float x) {
+    float t = 1.f + x * x;
+    return 0.5f / (t * std::sqrt(t));
+}
+inline float sigmoid_inv(float y) {
+    y = std::clamp(y, 1e-4f, 1.f - 1e-4f);
+    float u = 2.f * y - 1.f;
+    return u / std::sqrt(1.f - u * u);
+}
+struct Fit { float c0 = 0.f, c1 = 0.f, c2 = 0.f, scale = 0.f; };
+inline float eval_fit(const Fit &f, float lambda) {
+    constexpr float mid = 0.5f * (kLambdaMin + kLambdaMax);
+    constexpr float half = 0.5f * (kLambdaMax - kLambdaMin);
+    float L = (lambda - mid) / half;
+    float p = f.c0 + L * (f.c1 + L * f.c2);
+    return sigmoid(p) * f.scale;
+}
+bool solve3(const float J[3][3], const float b[3], float x[3]) {
+    float det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (std::abs(det) < 1e-12f) return false;
+    float inv = 1.f / det;
+    float d0 = b[0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (b[1] * J[2][2] - J[1][2] * b[2]) +
+               J[0][2] * (b[1] * J[2][1] - J[1][1] * b[2]);
+    float d1 = J[0][0] * (b[1] * J[2][2] - J[1][2] * b[2]) - b[0] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * b[2] - b[1] * J[2][0]);
+    float d2 = J[0][0] * (J[1][1] * b[2] - b[1] * J[2][1]) - J[0][1] * (J[1][0] * b[2] - b[1] * J[2][0]) +
+               b[0] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    x[0] = d0 * inv;
+    x[1] = d1 * inv;
+    x[2] = d2 * inv;
+    return true;
+}
+void newton(const V3 &target, const float cin[3], float cout_[3]) {
+    constexpr float mid = 0.5f * (kLambdaMin + kLambdaMax);
+    constexpr float half = 0.5f * (kLambdaMax - kLambdaMin);
+    float c[3] = {cin[0], cin[1], cin[2]};
+    for (int iter = 0; iter < 15; iter++) {
+        float xyz[3] = {0.f, 0.f, 0.f};
+        float J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int i = 0; i < kSamples; i++) {
+            float l = lambda_at(i);
+            float ln = (l - mid) / half;
+            float p = c[0] + ln * (c[1] + ln * c[2]);
+            float s = sigmoid(p), ds = sigmoid_d(p);
+            float xb = xbar(l), yb = ybar(l), zb = zbar(l);
+            xyz[0] += s * xb;
+            xyz[1] += s * yb;
+            xyz[2] += s * zb;
+            float ln2 = ln * ln;
+            J[0][0] += ds * xb;
+            J[0][1] += ds * ln * xb;
+            J[0][2] += ds * ln2 * xb;
+            J[1][0] += ds * yb;
+            J[1][1] += ds * ln * yb;
+            J[1][2] += ds * ln2 * yb;
+            J[2][0] += ds * zb;
+            J[2][1] += ds * ln * zb;
+            J[2][2] += ds * ln2 * zb;
+        }
+        float dl = kStep;
+        for (int k = 0; k < 3; k++) {
+            xyz[k] *= dl;
+            for (int j = 0; j < 3; j++) J[k][j] *= dl;
+        }
+        float res[3] = {xyz[0] - target.x, xyz[1] - target.y, xyz[2] - target.z};
+        float rn = std::sqrt(res[0] * res[0] + res[1] * res[1] + res[2] * res[2]);
+        if (rn < 1e-6f) break;
+        float delta[3];
+        if (!solve3(J, res, delta)) break;
+        float prev[3] = {c[0], c[1], c[2]};
+        float step = 1.f;
+        bool reduced = false;
+        for (int trial = 0; trial < 4; trial++) {
+            c[0] = prev[0] - step * delta[0];
+            c[1] = prev[1] - step * delta[1];
+            c[2] = prev[2] - step * delta[2];
+            float t3[3] = {0.f, 0.f, 0.f};
+            for (int i = 0; i < kSamples; i++) {
+                float l = lambda_at(i);
+                float ln = (l - mid) / half;
+                float p = c[0] + ln * (c[1] + ln * c[2]);
+                float s = sigmoid(p);
+                t3[0] += s * xbar(l);
+                t3[1] += s * ybar(l);
+                t3[2] += s * zbar(l);
+            }
+            t3[0] *= dl;
+            t3[1] *= dl;
+            t3[2] *= dl;
+            float tr[3] = {t3[0] - target.x, t3[1] - target.y, t3[2] - target.z};
+            float tn = std::sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2]);
+            if (tn < rn) { reduced = true; break; }
+            step *= 0.5f;
+        }
+        if (!reduced) {
+            c[0] = prev[0];
+            c[1] = prev[1];
+            c[2] = prev[2];
+            break;
+        }
+    }
+    cout_[0] = c[0];
+    cout_[1] = c[1];
+    cout_[2] = c[2];
+}
+float spikiness(const float c[3]) {
+    // Spectrum::fromSigmoidCoefficients (Spectrum.h:55-67) then max/median.
+    constexpr float mid = 0.5f * (kLambdaMin + kLambdaMax);
+    constexpr float half = 0.5f * (kLambdaMax - kLambdaMin);
+    float samples[kSamples];
+    float mx = 0.f;
+    for (int i = 0; i < kSamples; i++) {
+        float ln = (lambda_at(i) - mid) / half;
+        float p = c[0] + ln * (c[1] + ln * c[2]);
+        samples[i] = 0.5f + p / (2.f * std::sqrt(1.f + p * p));
+        if (samples[i] > mx) mx = samples[i];
+    }
+    std::nth_element(samples, samples + kSamples / 2, samples + kSamples);
+    float median = samples[kSamples / 2];
+    if (median < 1e-8f) return (mx < 1e-8f) ? 1.f : 1e30f;
+    return mx / median;
+}
+void fit_coefficients(const V3 &target, float &c0, float &c1, float &c2) {
+    constexpr float kXn = 0.95046f, kZn = 1.08906f;
+    V3 gray = v3(target.y * kXn, target.y, target.y * kZn);
+    float c[3] = {sigmoid_inv(target.y / ybar_integral()), 0.f, 0.f};
+    float good[3] = {c[0], c[1], c[2]};
+    for (int step = 1; step <= 32; step++) {
+        float t = (float)step / (float)32;
+        V3 cur = v3((1.f - t) * gray.x + t * target.x, (1.f - t) * gray.y + t * target.y,
+                    (1.f - t) * gray.z + t * target.z);
+        float nxt[3];
+        newton(cur, c, nxt);
+        if (spikiness(nxt) > 200.f) break;
+        c[0] = nxt[0];
+        c[1] = nxt[1];
+        c[2] = nxt[2];
+        good[0] = c[0];
+        good[1] = c[1];
+        good[2] = c[2];
+    }
+    c0 = good[0];
+    c1 = good[1];
+    c2 = good[2];
+}
+Fit fit_sigmoid(V3 rgb) {
+    float r = std::clamp(rgb.x, 0.f, 1.f), g = std::clamp(rgb.y, 0.f, 1.f), b = std::clamp(rgb.z, 0.f, 1.f);
+    if (r <= 0.f && g <= 0.f && b <= 0.f) return Fit{};
+    if (r >= 1.f && g >= 1.f && b >= 1.f) {
+        float scale = ybar_integral();
+        Fit f;
+        f.c0 = sigmoid_inv(1.f / scale);
+        f.scale = scale;
+        return f;
+    }
+    V3 xyz = srgb_to_xyz(v3(r, g, b));
+    Fit f;
+    fit_coefficients(xyz, f.c0, f.c1, f.c2);
+    f.scale = ybar_integral();
+    return f;
+}
+
+// --------------------------------------------------------------- Material ----
+// code/Includes/Material.h:8-139
+struct Mat {
+    V3 albedo, emissive;
+    float tabA[kSamples] = {}, tabE[kSamples] = {};
+    bool useTabA = false, useTabE = false;
+    bool metallic = false, transparent = false;
+    float ior = 1.5f, cauchyB = 0.f;
+    Fit fitA, fitE;
+    float albedoAt(float l) const {
+        if (useTabA) return spec_eval(tabA, l);
+        return std::min(eval_fit(fitA, l), 1.f);
+    }
+    float emissiveAt(float l) const {
+        if (useTabE) return spec_eval(tabE, l);
+        return eval_fit(fitE, l);
+    }
+    bool isEmissive() const { return emissive.x > 0 || emissive.y > 0 || emissive.z > 0; }
+    void populate() {
+        fitA = fit_sigmoid(albedo);
+        if (isEmissive()) {
+            float maxE = std::max({emissive.x, emissive.y, emissive.z});
+            fitE = fit_sigmoid(v3(emissive.x / maxE, emissive.y / maxE, emissive.z / maxE));
+            fitE.scale *= maxE;
+        } else {
+            fitE = Fit{};
+        }
+    }
+};
+
+// ------------------------------------------------------------ primitives ----
+struct Sph { V3 c; float r; int mat; };
+// code/Includes/Sphere.h:20-39
+inline bool sphere_hit(const Sph &s, V3 o, V3 d, float &t0) {
+    V3 cp = s.c - o;
+    float rl = dot(d, cp);
+    float tSq = dot(cp, cp) - rl * rl;
+    if (tSq > s.r * s.r) return false;
+    float td = std::sqrt(s.r * s.r - tSq);
+    t0 = rl - td;
+    float t1 = rl + td;
+    if (t0 < 0) t0 = t1;
+    if (t0 < 0) return false;
+    return true;
+}
+struct Pln { V3 o, u, v, N; float area; int mat; };
+inline Pln make_plane(V3 o, V3 u, V3 v, int mat) {
+    Pln p;
+    p.o = o; p.u = u; p.v = v; p.mat = mat;
+    V3 cr = cross(u, v);
+    p.area = length(cr);
+    p.N = normalize(cr);
+    return p;
+}
+// code/Includes/Plane.h:33-54
+inline bool plane_hit(const Pln &p, V3 o, V3 d, V3 &hit, float &t0, float closest) {
+    const float EPS = 1e-6f;
+    float denom = dot(d, p.N);
+    if (std::fabs(denom) <= EPS) return false;
+    t0 = dot(p.o - o, p.N) / denom;
+    if (t0 <= EPS || t0 > closest) return false;
+    V3 th = o + d * t0;
+    V3 lh = th - p.o;
+    float s = dot(lh, p.u) / dot(p.u, p.u);
+    float q = dot(lh, p.v) / dot(p.v, p.v);
+    if (s > 1 || s < 0 || q > 1 || q < 0) return false;
+    hit = th;
+    return true;
+}
+struct Tri {
+    V3 v0, v1, v2, n0, n1, n2, flatN;
+    bool smooth = false;
+    int mat = 0;
+    int orig = 0;  // insertion index (bookkeeping only)
+};
+inline Tri make_tri(V3 a, V3 b, V3 c, int mat) {
+    Tri t;
+    t.v0 = a; t.v1 = b; t.v2 = c; t.mat = mat;
+    t.flatN = normalize(cross(b - a, c - a));
+    return t;
+}
+// code/Includes/Triangle.h:53-92
+inline bool tri_hit(const Tri &tr, V3 o, V3 d, V3 &hit, V3 &N, float &t0, float closest) {
+    const float EPS = 1e-6f;
+    V3 e1 = tr.v1 - tr.v0;
+    V3 e2 = tr.v2 - tr.v0;
+    V3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (std::fabs(det) < EPS) return false;
+    float inv = 1.f / det;
+    V3 tv = o - tr.v0;
+    float u = dot(tv, pv) * inv;
+    if (u < 0.f || u > 1.f) return false;
+    V3 qv = cross(tv, e1);
+    float v = dot(d, qv) * inv;
+    if (v < 0.f || u + v > 1.f) return false;
+    float t = dot(e2, qv) * inv;
+    if (t <= EPS || t >= closest) return false;
+    t0 = t;
+    hit = o + d * t;
+    if (tr.smooth) {
+        float w = 1.f - u - v;
+        V3 in = v3(tr.n0.x * w + tr.n1.x * u + tr.n2.x * v, tr.n0.y * w + tr.n1.y * u + tr.n2.y * v,
+                   tr.n0.z * w + tr.n1.z * u + tr.n2.z * v);
+        N = normalize(in);
+    } else {
+        N = tr.flatN;
+    }
+    return true;
+}
+
+// -------------------------------------------------------------------- BVH ----
+// code/Bvh/Bvh.cpp:29-325 (16-bin SAH, preorder nodes, in-place std::partition).
+struct Node { float bmin[3], bmax[3]; int left, right, count; };
+struct AABB {
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void add(V3 p) {
+        for (int i = 0; i < 3; i++) {
+            if (p[i] < mn[i]) mn[i] = p[i];
+            if (p[i] > mx[i]) mx[i] = p[i];
+        }
+    }
+    void add(const Tri &t) { add(t.v0); add(t.v1); add(t.v2); }
+    void merge(const AABB &o) {
+        for (int i = 0; i < 3; i++) {
+            if (o.mn[i] < mn[i]) mn[i] = o.mn[i];
+            if (o.mx[i] > mx[i]) mx[i] = o.mx[i];
+        }
+    }
+    float area() const {
+        float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.f;
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+inline float centroid(const Tri &t, int a) { return (t.v0[a] + t.v1[a] + t.v2[a]) * (1.f / 3.f); }
+inline V3 centroidv(const Tri &t) { return v3(centroid(t, 0), centroid(t, 1), centroid(t, 2)); }
+
+struct Split { int axis = -1, bin = -1; float cost = FLT_MAX; };
+Split find_split(const std::vector<Tri> &tris, int b, int e, const AABB &box) {
+    AABB cb;
+    for (int i = b; i < e; i++) cb.add(centroidv(tris[i]));
+    Split best;
+    float parentSA = box.area();
+    if (parentSA <= 0.f) return best;
+    for (int axis = 0; axis < 3; axis++) {
+        float lo = cb.mn[axis], hi = cb.mx[axis];
+        float ext = hi - lo;
+        if (ext < 1e-6f) continue;
+        float scale = (float)16 / ext;
+        AABB bb[16];
+        int bc[16] = {};
+        for (int i = b; i < e; i++) {
+            float c = centroid(tris[i], axis);
+            int idx = std::clamp((int)((c - lo) * scale), 0, 15);
+            bb[idx].add(tris[i]);
+            bc[idx]++;
+        }
+        float lSA[15], rSA[15];
+        int lC[15], rC[15];
+        AABB acc;
+        int ac = 0;
+        for (int i = 0; i < 15; i++) {
+            acc.merge(bb[i]);
+            ac += bc[i];
+            lSA[i] = acc.area();
+            lC[i] = ac;
+        }
+        acc = AABB{};
+        ac = 0;
+        for (int i = 15; i > 0; i--) {
+            acc.merge(bb[i]);
+            ac += bc[i];
+            rSA[i - 1] = acc.area();
+            rC[i - 1] = ac;
+        }
+        for (int s = 0; s < 15; s++) {
+            if (lC[s] == 0 || rC[s] == 0) continue;
+            float cost = 1.f + 2.f * (lSA[s] * lC[s] + rSA[s] * rC[s]) / parentSA;
+            if (cost < best.cost) {
+                best.cost = cost;
+                best.axis = axis;
+                best.bin = s;
+            }
+        }
+    }
+    return best;
+}
+int partition_sah(std::vector<Tri> &tris, int b, int e, int axis, int bin) {
+    AABB cb;
+    for (int i = b; i < e; i++) cb.add(centroidv(tris[i]));
+    float lo = cb.mn[axis], hi = cb.mx[axis];
+    float scale = (float)16 / (hi - lo);
+    auto it = std::partition(tris.begin() + b, tris.begin() + e, [&](const Tri &t) {
+        float c = centroid(t, axis);
+        int idx = std::clamp((int)((c - lo) * scale), 0, 15);
+        return idx <= bin;
+    });
+    return (int)(it - tris.begin());
+}
+int build_rec(std::vector<Tri> &tris, std::vector<Node> &nodes, int b, int e) {
+    int ni = (int)nodes.size();
+    nodes.emplace_back();
+    AABB box;
+    for (int i = b; i < e; i++) box.add(tris[i]);
+    for (int i = 0; i < 3; i++) {
+        nodes[ni].bmin[i] = box.mn[i];
+        nodes[ni].bmax[i] = box.mx[i];
+    }
+    int count = e - b;
+    auto leaf = [&]() {
+        nodes[ni].left = b;
+        nodes[ni].right = 0;
+        nodes[ni].count = count;
+        return ni;
+    };
+    if (count <= 4) return leaf();
+    Split sp = find_split(tris, b, e, box);
+    float leafCost = (float)count * 2.f;
+    if (sp.axis < 0 || sp.cost >= leafCost) return leaf();
+    int mid = partition_sah(tris, b, e, sp.axis, sp.bin);
+    if (mid == b || mid == e) return leaf();
+    int l = build_rec(tris, nodes, b, mid);
+    int r = build_rec(tris, nodes, mid, e);
+    nodes[ni].left = l;
+    nodes[ni].right = r;
+    nodes[ni].count = 0;
+    return ni;
+}
+std::vector<Node> bvh_build(std::vector<Tri> &tris) {
+    std::vector<Node> nodes;
+    if (tris.empty()) return nodes;
+    nodes.reserve(2 * tris.size());
+    build_rec(tris, nodes, 0, (int)tris.size());
+    return nodes;
+}
+// Bvh.cpp:294-311
+inline bool ray_aabb(V3 o, V3 d, const float *mn, const float *mx, float segMax, float &tNear) {
+    float tmin = 0.f, tmax = segMax;
+    for (int i = 0; i < 3; i++) {
+        float invD = 1.f / d[i];
+        float t1 = (mn[i] - o[i]) * invD;
+        float t2 = (mx[i] - o[i]) * invD;
+        if (t1 > t2) std::swap(t1, t2);
+        tmin = std::max(tmin, t1);
+        tmax = std::min(tmax, t2);
+        if (tmin > tmax) return false;
+    }
+    tNear = tmin;
+    return true;
+}
+// Bvh.cpp:353-444 (64-entry stack, overflowing pushes dropped)
+bool bvh_hit(const std::vector<Node> &nodes, const std::vector<Tri> &tris, V3 o, V3 d, V3 &hit, V3 &N, int &mat,
+             int &triIdxOut, float &tout, float closest_t) {
+    if (nodes.empty()) return false;
+    struct E { int idx; float tn; };
+    E st[64];
+    int top = 0;
+    {
+        float tr;
+        if (!ray_aabb(o, d, nodes[0].bmin, nodes[0].bmax, closest_t, tr)) return false;
+        st[top++] = {0, tr};
+    }
+    bool any = false;
+    float closest = closest_t;
+    while (top > 0) {
+        E en = st[--top];
+        if (en.tn > closest) continue;
+        const Node &n = nodes[en.idx];
+        if (n.count > 0) {
+            V3 th, tn;
+            float tt;
+            for (int i = 0; i < n.count; i++) {
+                int ti = n.left + i;
+                if (!tri_hit(tris[ti], o, d, th, tn, tt, closest)) continue;
+                closest = tt;
+                hit = th;
+                N = tn;
+                mat = tris[ti].mat;
+                triIdxOut = ti;
+                any = true;
+            }
+            continue;
+        }
+        const Node &cl = nodes[n.left];
+        const Node &cr = nodes[n.right];
+        float tL, tR;
+        bool hL = ray_aabb(o, d, cl.bmin, cl.bmax, closest, tL);
+        bool hR = ray_aabb(o, d, cr.bmin, cr.bmax, closest, tR);
+        if (hL && hR) {
+            if (top + 2 > 64) continue;
+            if (tL <= tR) {
+                st[top++] = {n.right, tR};
+                st[top++] = {n.left, tL};
+            } else {
+                st[top++] = {n.left, tL};
+                st[top++] = {n.right, tR};
+            }
+        } else if (hL) {
+            if (top + 1 > 64) continue;
+            st[top++] = {n.left, tL};
+        } else if (hR) {
+            if (top + 1 > 64) continue;
+            st[top++] = {n.right, tR};
+        }
+    }
+    if (any) tout = closest;
+    return any;
+}
+int bvh_depth(const std::vector<Node> &nodes, int idx) {
+    const Node &n = nodes[idx];
+    if (n.count > 0) return 1;
+    return 1 + std::max(bvh_depth(nodes, n.left), bvh_depth(nodes, n.right));
+}
+
+// ----------------------------------------------------------------- Optics ----
+// code/Includes/Optics.h:22-82
+inline float schlick(float cosT, float n1, float n2) {
+    float F0 = (n1 - n2) / (n1 + n2);
+    F0 *= F0;
+    return F0 + (1.f - F0) * std::pow(1.f - cosT, 5.f);
+}
+inline float cauchy_ior(float ior, float B, float l) { return ior + B * 1e4f / (l * l); }
+struct Bounce { V3 dir, origin; };
+inline Bounce dielectric(V3 rd, V3 N, V3 hit, bool entering, float ior, float u) {
+    float cosI = -dot(rd, N);
+    float n1 = entering ? 1.f : ior;
+    float n2 = entering ? ior : 1.f;
+    float eta = n1 / n2;
+    float sinT2 = eta * eta * (1.f - cosI * cosI);
+    if (sinT2 >= 1.f) return {rd + N * (2.f * cosI), hit + N * 1e-3f};
+    float F = schlick(cosI, n1, n2);
+    if (u < F) return {rd + N * (2.f * cosI), hit + N * 1e-3f};
+    float cosT = std::sqrt(1.f - sinT2);
+    return {rd * eta + N * (eta * cosI - cosT), hit - N * 1e-3f};
+}
+// IEEE-only variant for the non-branching estimator: Schlick's (1-c)^5 as
+// explicit products instead of powf.
+inline float fm_schlick(float cosT, float n1, float n2) {
+    float F0 = (n1 - n2) / (n1 + n2);
+    F0 *= F0;
+    float m = 1.f - cosT;
+    float m2 = m * m;
+    return F0 + (1.f - F0) * (m2 * m2 * m);
+}
+inline Bounce fm_dielectric(V3 rd, V3 N, V3 hit, bool entering, float ior, float u) {
+    float cosI = -dot(rd, N);
+    float n1 = entering ? 1.f : ior;
+    float n2 = entering ? ior : 1.f;
+    float eta = n1 / n2;
+    float sinT2 = eta * eta * (1.f - cosI * cosI);
+    if (sinT2 >= 1.f) return {rd + N * (2.f * cosI), hit + N * kEps3};
+    float F = fm_schlick(cosI, n1, n2);
+    if (u < F) return {rd + N * (2.f * cosI), hit + N * kEps3};
+    float cosT = std::sqrt(1.f - sinT2);
+    return {rd * eta + N * (eta * cosI - cosT), hit - N * kEps3};
+}
+
+// ------------------------------------------------------------------ Scene ----
+enum class LK { Plane, TriSet };
+struct Light {
+    LK kind = LK::Plane;
+    Pln plane;
+    std::vector<Tri> tris;
+    std::vector<float> cum;
+    float area = 0.f;
+};
+struct Scene {
+    std::vector<Mat> mats;
+    std::vector<Sph> spheres;
+    std::vector<Pln> walls;
+    std::vector<Tri> tris;     // leaf order after finalize
+    std::vector<Node> nodes;
+    std::vector<Light> lights;
+    std::vector<Pln> planes;   // render order: light planes, then walls
+    V3 cam;
+    float fov = 65.f;
+    float totalArea = 0.f;
+    bool finalized = false;
+};
+
+Light make_triset_light(std::vector<Tri> tris) {
+    Light L;
+    L.kind = LK::TriSet;
+    L.tris = std::move(tris);
+    float acc = 0.f;
+    for (const auto &t : L.tris) {
+        V3 cr = cross(t.v1 - t.v0, t.v2 - t.v0);
+        float a = 0.5f * length(cr);
+        acc += a;
+        L.cum.push_back(acc);
+    }
+    L.area = acc;
+    return L;
+}
+
+struct Hit {
+    bool ok = false;
+    V3 p, N;
+    int mat = -1;
+    float t = FLT_MAX;
+    int kind = -1, prim = -1;
+};
+// Renderer::sceneIntersect (code/Renderer.cpp:891-949)
+inline Hit scene_hit(const Scene &s, V3 o, V3 d, float tmax = FLT_MAX) {
+    Hit h;
+    float closest = tmax;
+    float t0 = 0.f;
+    V3 hit, N;
+    int mat = -1;
+    for (size_t i = 0; i < s.spheres.size(); i++) {
+        const Sph &sp = s.spheres[i];
+        if (!sphere_hit(sp, o, d, t0) || t0 >= closest) continue;
+        closest = t0;
+        hit = o + d * t0;
+        N = normalize(hit - sp.c);
+        mat = sp.mat;
+        h.kind = 0;
+        h.prim = (int)i;
+    }
+    for (size_t i = 0; i < s.planes.size(); i++) {
+        const Pln &pl = s.planes[i];
+        if (!plane_hit(pl, o, d, hit, t0, closest) || t0 >= closest) continue;
+        closest = t0;
+        N = pl.N;
+        mat = pl.mat;
+        h.kind = 1;
+        h.prim = (int)i;
+    }
+    if (!s.nodes.empty()) {
+        V3 th, tn;
+        int tm = -1, ti = -1;
+        float tt;
+        if (bvh_hit(s.nodes, s.tris, o, d, th, tn, tm, ti, tt, closest)) {
+            closest = tt;
+            hit = th;
+            N = tn;
+            mat = tm;
+            h.kind = 2;
+            h.prim = s.tris[ti].orig;
+        }
+    }
+    h.ok = closest < tmax;
+    h.t = closest;
+    h.p = hit;
+    h.N = N;
+    h.mat = mat;
+    return h;
+}
+
+// ------------------------------------------------------ branched estimator ----
+// Restatement of Renderer::castRay / castRaySpectral with a single NumGen
+// stream (seeded render => one thread, Renderer.cpp:158-160).
+struct Branched {
+    const Scene &S;
+    const orc_branched_params &P;
+    Rng rng;
+    float eps() { return rng.next(); }
+
+    V3 cast_rgb(V3 ro, V3 rd, int depth) {
+        if (depth >= P.depth) return V3{};
+        Hit h = scene_hit(S, ro, rd);
+        if (!h.ok) return V3{};
+        V3 hit = h.p, N = h.N;
+        const Mat &m = S.mats[h.mat];
+        bool entering = dot(rd, N) < 0.f;
+        if (!entering) N = N * -1.f;
+        if (m.isEmissive()) return m.emissive;
+        if (m.metallic) {
+            float cosI = -dot(rd, N);
+            V3 refl = rd + N * (2.f * cosI);
+            hit = hit + N * 1e-3f;
+            V3 r = cast_rgb(hit, refl, depth + 1);
+            return v3(r.x * m.albedo.x, r.y * m.albedo.y, r.z * m.albedo.z);
+        }
+        if (m.transparent) {
+            Bounce b = dielectric(rd, N, hit, entering, m.ior, eps());
+            V3 r = cast_rgb(b.origin, b.dir, depth + 1);
+            return v3(r.x * m.albedo.x, r.y * m.albedo.y, r.z * m.albedo.z);
+        }
+        V3 ind;
+        const int strata = P.stratified ? std::max(1, (int)std::round(std::sqrt((float)P.samples))) : 0;
+        for (int i = 0; i < P.samples; i++) {
+            float r1, r2;
+            if (P.stratified) {
+                int sx = i % strata, sy = (i / strata) % strata;
+                r1 = (sx + eps()) / (float)strata;
+                r2 = (sy + eps()) / (float)strata;
+            } else {
+                r1 = eps();
+                r2 = eps();
+            }
+            hit = hit + N * kEps3;  // Vec3f::operator+ mutates `hit` (Renderer.cpp:545)
+            V3 dir = hemisphere(N, r1, r2);
+            if (P.russian && depth >= 1) {
+                float maxA = std::max({m.albedo.x, m.albedo.y, m.albedo.z});
+                float p = std::min(0.95f, std::max(0.05f, maxA));
+                if (eps() > p) continue;
+                V3 r = cast_rgb(hit, dir, depth + 1);
+                ind = ind + mul(r, m.albedo) / p;
+            } else {
+                V3 r = cast_rgb(hit, dir, depth + 1);
+                ind = ind + mul(r, m.albedo);
+            }
+        }
+        ind = ind / (float)P.samples;
+        V3 dl;
+        if (S.totalArea > 0.f) {
+            for (int i = 0; i < P.shadow; i++) {
+                V3 sP, sN, sE;
+                sample_light(sP, sN, nullptr, &sE);
+                V3 Li = sP - hit;
+                V3 wi = normalize(Li);
+                float cosT = std::max(0.f, dot(wi, N));
+                float d2 = dot(Li, Li);
+                V3 so;
+                if (cosT <= 0) {
+                    so = hit - N * kEps3;
+                } else {
+                    hit = hit + N * kEps3;  // mutating branch of the ternary (Renderer.cpp:622)
+                    so = hit;
+                }
+                Hit sh = scene_hit(S, so, wi);
+                bool inShadow = sh.ok && (double)d2 - 1e-3 > (double)dot(sh.p - so, sh.p - so) &&
+                                !S.mats[sh.mat].isEmissive();
+                if (!inShadow) {
+                    float cosL = std::max(0.f, dot(sN, Li * -1.f));
+                    float G = (cosT * cosL) / d2;
+                    V3 dc = mul(m.albedo / kPiF, sE) * G * S.totalArea;
+                    if (P.mis && cosL > 1e-6f) {
+                        float pl = d2 / (cosL * S.totalArea);
+                        float pb = cosT / kPiF;
+                        float w = (pl * pl) / (pl * pl + pb * pb);
+                        dc = dc * w;
+                    }
+                    dl = dl + dc;
+                }
+            }
+        }
+        return dl / (float)P.shadow + ind;
+    }
+
+    using SS = std::array<float, 4>;
+    SS cast_spec(V3 ro, V3 rd, int depth, const SS &lam) {
+        SS zero{};
+        if (depth >= P.depth) return zero;
+        Hit h = scene_hit(S, ro, rd);
+        if (!h.ok) return zero;
+        V3 hit = h.p, N = h.N;
+        const Mat &m = S.mats[h.mat];
+        bool entering = dot(rd, N) < 0.f;
+        if (!entering) N = N * -1.f;
+        if (m.isEmissive()) {
+            SS e;
+            for (int k = 0; k < 4; k++) e[k] = m.emissiveAt(lam[k]);
+            return e;
+        }
+        if (m.metallic) {
+            float cosI = -dot(rd, N);
+            V3 refl = rd + N * (2.f * cosI);
+            hit = hit + N * 1e-3f;
+            SS r = cast_spec(hit, refl, depth + 1, lam);
+            SS o;
+            for (int k = 0; k < 4; k++) o[k] = r[k] * m.albedoAt(lam[k]);
+            return o;
+        }
+        if (m.transparent) {
+            if (m.cauchyB == 0.f) {
+                Bounce b = dielectric(rd, N, hit, entering, m.ior, eps());
+                SS r = cast_spec(b.origin, b.dir, depth + 1, lam);
+                SS o;
+                for (int k = 0; k < 4; k++) o[k] = r[k] * m.albedoAt(lam[k]);
+                return o;
+            }
+            SS o;
+            for (int k = 0; k < 4; k++) {
+                float iorK = cauchy_ior(m.ior, m.cauchyB, lam[k]);
+                Bounce b = dielectric(rd, N, hit, entering, iorK, eps());
+                SS single;
+                single.fill(lam[k]);
+                SS r = cast_spec(b.origin, b.dir, depth + 1, single);
+                o[k] = r[0] * m.albedoAt(lam[k]);
+            }
+            return o;
+        }
+        SS alb;
+        for (int k = 0; k < 4; k++) alb[k] = m.albedoAt(lam[k]);
+        SS ind{};
+        const int strata = P.stratified ? std::max(1, (int)std::round(std::sqrt((float)P.samples))) : 0;
+        for (int i = 0; i < P.samples; i++) {
+            float r1, r2;
+            if (P.stratified) {
+                int sx = i % strata, sy = (i / strata) % strata;
+                r1 = (sx + eps()) / (float)strata;
+                r2 = (sy + eps()) / (float)strata;
+            } else {
+                r1 = eps();
+                r2 = eps();
+            }
+            hit = hit + N * kEps3;
+            V3 dir = hemisphere(N, r1, r2);
+            if (P.russian && depth >= 1) {
+                float p = std::min(0.95f, std::max(0.05f, alb[0]));
+                if (eps() > p) continue;
+                SS r = cast_spec(hit, dir, depth + 1, lam);
+                for (int k = 0; k < 4; k++) ind[k] += r[k] * alb[k] / p;
+            } else {
+                SS r = cast_spec(hit, dir, depth + 1, lam);
+                for (int k = 0; k < 4; k++) ind[k] += r[k] * alb[k];
+            }
+        }
+        for (int k = 0; k < 4; k++) ind[k] /= (float)P.samples;
+        SS dl{};
+        if (S.totalArea > 0.f) {
+            for (int i = 0; i < P.shadow; i++) {
+                V3 sP, sN;
+                const Mat *lm = nullptr;
+                sample_light(sP, sN, &lm, nullptr);
+                V3 Li = sP - hit;
+                V3 wi = normalize(Li);
+                float cosT = std::max(0.f, dot(wi, N));
+                float d2 = dot(Li, Li);
+                V3 so;
+                if (cosT <= 0) {
+                    so = hit - N * kEps3;
+                } else {
+                    hit = hit + N * kEps3;
+                    so = hit;
+                }
+                Hit sh = scene_hit(S, so, wi);
+                bool inShadow = sh.ok && (double)d2 - 1e-3 > (double)dot(sh.p - so, sh.p - so) &&
+                                !S.mats[sh.mat].isEmissive();
+                if (!inShadow) {
+                    float cosL = std::max(0.f, dot(sN, Li * -1.f));
+                    float G = (cosT * cosL) / d2;
+                    float w = 1.f;
+                    if (P.mis && cosL > 1e-6f) {
+                        float pl = d2 / (cosL * S.totalArea);
+                        float pb = cosT / kPiF;
+                        w = (pl * pl) / (pl * pl + pb * pb);
+                    }
+                    for (int k = 0; k < 4; k++) {
+                        float eL = lm->emissiveAt(lam[k]);
+                        float c = (alb[k] / kPiF) * eL * G * S.totalArea * w;
+                        dl[k] += c;
+                    }
+                }
+            }
+        }
+        SS o;
+        for (int k = 0; k < 4; k++) o[k] = dl[k] / (float)P.shadow + ind[k];
+        return o;
+    }
+
+    // Ray::genRayFromIntersection (code/Includes/Ray.h:28-53), glibc cosf/sinf.
+    static V3 hemisphere(V3 N, float r1, float r2) {
+        V3 helper = std::abs(N.x) <= std::abs(N.y) ? v3(1, 0, 0) : v3(0, 1, 0);
+        V3 T = normalize(cross(N, helper));
+        V3 B = cross(N, T);
+        float r = std::sqrt(r1);
+        float phi = 2.f * kPiF * r2;
+        float x = r * std::cos(phi);
+        float y = r * std::sin(phi);
+        float z = std::sqrt(std::max(0.f, 1.f - x * x - y * y));
+        return T * x + B * y + N * z;
+    }
+    // Renderer.cpp:570-615 light pick + area sample
+    void sample_light(V3 &sP, V3 &sN, const Mat **lm, V3 *sE) {
+        const Light *pk = &S.lights.front();
+        if (S.lights.size() > 1) {
+            float target = eps() * S.totalArea;
+            float cum = 0.f;
+            for (const auto &L : S.lights) {
+                cum += L.area;
+                if (target <= cum) { pk = &L; break; }
+            }
+        }
+        if (pk->kind == LK::Plane) {
+            const Pln &p = pk->plane;
+            float ru = eps(), rv = eps();
+            sP = p.o + p.u * ru + p.v * rv;
+            sN = p.N;
+            if (lm) *lm = &S.mats[p.mat];
+            if (sE) *sE = S.mats[p.mat].emissive;
+        } else {
+            float rt = eps() * pk->area;
+            auto it = std::lower_bound(pk->cum.begin(), pk->cum.end(), rt);
+            int ti = std::min((int)(it - pk->cum.begin()), (int)pk->tris.size() - 1);
+            const Tri &t = pk->tris[ti];
+            float r1 = eps(), r2 = eps();
+            if (r1 + r2 > 1.f) { r1 = 1.f - r1; r2 = 1.f - r2; }
+            sP = t.v0 + (t.v1 - t.v0) * r1 + (t.v2 - t.v0) * r2;
+            sN = t.flatN;
+            if (lm) *lm = &S.mats[t.mat];
+            if (sE) *sE = S.mats[t.mat].emissive;
+        }
+    }
+};
+
+// -------------------------------------------------- non-branching estimator ----
+// One path per sample, semantics of OpenglRenderer.cpp tracePath (:783-911),
+// tracePathSpectral (:1060-1247) and tracePathSpectralSingle (:930-1041),
+// with: per-path NumGen streams, CPU material evaluation (Material.h:92-101),
+// CPU sceneIntersect tie rules, and the IEEE-only math set above.
+struct PathCtx {
+    const Scene &S;
+    const orc_path_params &P;
+    float scale, aspect;
+    uint64_t n_ext = 0, n_shadow = 0, n_emit = 0;
+};
+
+struct PathState {
+    V3 o, d;
+    float tp[4];
+    float lam[4];
+    int nch;       // active channels
+    int ch0;       // first output channel written (forked sub-path: its channel)
+    int bounce;
+    bool first;
+    Rng rng;
+};
+
+inline V3 fm_hemisphere(V3 N, float r1, float r2) {
+    V3 helper = std::fabs(N.x) <= std::fabs(N.y) ? v3(1, 0, 0) : v3(0, 1, 0);
+    V3 T = normalize(cross(N, helper));
+    V3 B = cross(N, T);
+    float r = std::sqrt(r1);
+    float sn, cs;
+    fm_sincos_turns(r2, sn, cs);
+    float x = r * cs;
+    float y = r * sn;
+    float z = std::sqrt(std::max(0.f, 1.f - x * x - y * y));
+    return T * x + B * y + N * z;
+}
+
+void trace_path(PathCtx &C, PathState ps, float pr1, float pr2, float rad[4]) {
+    const Scene &S = C.S;
+    const orc_path_params &P = C.P;
+    const bool spectral = P.spectral != 0;
+    while (ps.bounce < P.depth) {
+        C.n_ext++;
+        Hit h = scene_hit(S, ps.o, ps.d);
+        if (!h.ok) return;
+        V3 hit = h.p, N = h.N;
+        const Mat &m = S.mats[h.mat];
+        bool entering = dot(ps.d, N) < 0.f;
+        if (!entering) N = N * -1.f;
+        if (m.isEmissive()) {
+            C.n_emit++;
+            for (int c = 0; c < ps.nch; c++) {
+                float le = spectral ? m.emissiveAt(ps.lam[c]) : m.emissive[c];
+                rad[ps.ch0 + c] += ps.tp[c] * le;
+            }
+            return;
+        }
+        float alb[4];
+        for (int c = 0; c < ps.nch; c++) alb[c] = spectral ? m.albedoAt(ps.lam[c]) : m.albedo[c];
+        if (m.metallic) {
+            float cosI = -dot(ps.d, N);
+            ps.d = ps.d + N * (2.f * cosI);
+            ps.o = hit + N * kEps3;
+            for (int c = 0; c < ps.nch; c++) ps.tp[c] *= alb[c];
+            ps.first = false;
+            ps.bounce++;
+            continue;
+        }
+        if (m.transparent) {
+            if (spectral && m.cauchyB != 0.f && ps.nch > 1) {
+                // Dispersion fork: one single-wavelength sub-path per hero channel,
+                // each with its own stream derived from the parent's state.
+                for (int c = 0; c < ps.nch; c++) {
+                    PathState ch = ps;
+                    ch.rng.state = stream_seed(ps.rng.state, (uint64_t)(c + 1));
+                    float iorc = cauchy_ior(m.ior, m.cauchyB, ps.lam[c]);
+                    Bounce b = fm_dielectric(ps.d, N, hit, entering, iorc, ch.rng.next());
+                    ch.o = b.origin;
+                    ch.d = b.dir;
+                    ch.tp[0] = ps.tp[c] * alb[c];
+                    ch.lam[0] = ps.lam[c];
+                    ch.nch = 1;
+                    ch.ch0 = ps.ch0 + c;
+                    ch.first = false;
+                    ch.bounce = ps.bounce + 1;
+                    trace_path(C, ch, pr1, pr2, rad);
+                }
+                return;
+            }
+            float ior = (spectral && ps.nch == 1) ? cauchy_ior(m.ior, m.cauchyB, ps.lam[0]) : m.ior;
+            Bounce b = fm_dielectric(ps.d, N, hit, entering, ior, ps.rng.next());
+            ps.d = b.dir;
+            ps.o = b.origin;
+            for (int c = 0; c < ps.nch; c++) ps.tp[c] *= alb[c];
+            ps.first = false;
+            ps.bounce++;
+            continue;
+        }
+        // diffuse: next-event estimation with S shadow rays
+        float dl[4] = {0.f, 0.f, 0.f, 0.f};
+        if (S.totalArea > 0.f) {
+            for (int i = 0; i < P.shadow; i++) {
+                const Light *pk = &S.lights.front();
+                if (S.lights.size() > 1) {
+                    float target = ps.rng.next() * S.totalArea;
+                    float cum = 0.f;
+                    for (const auto &L : S.lights) {
+                        cum += L.area;
+                        if (target <= cum) { pk = &L; break; }
+                    }
+                }
+                V3 sP, sN;
+                const Mat *lm;
+                if (pk->kind == LK::Plane) {
+                    float ru = ps.rng.next(), rv = ps.rng.next();
+                    sP = pk->plane.o + pk->plane.u * ru + pk->plane.v * rv;
+                    sN = pk->plane.N;
+                    lm = &S.mats[pk->plane.mat];
+                } else {
+                    float rt = ps.rng.next() * pk->area;
+                    auto it = std::lower_bound(pk->cum.begin(), pk->cum.end(), rt);
+                    int ti = std::min((int)(it - pk->cum.begin()), (int)pk->tris.size() - 1);
+                    const Tri &t = pk->tris[ti];
+                    float r1 = ps.rng.next(), r2 = ps.rng.next();
+                    if (r1 + r2 > 1.f) { r1 = 1.f - r1; r2 = 1.f - r2; }
+                    sP = t.v0 + (t.v1 - t.v0) * r1 + (t.v2 - t.v0) * r2;
+                    sN = t.flatN;
+                    lm = &S.mats[t.mat];
+                }
+                V3 Li = sP - hit;
+                V3 wi = normalize(Li);
+                float cosT = std::max(0.f, dot(wi, N));
+                float d2 = dot(Li, Li);
+                V3 so = cosT <= 0 ? hit - N * kEps3 : hit + N * kEps3;
+                C.n_shadow++;
+                Hit sh = scene_hit(S, so, wi);
+                bool occl = sh.ok && (double)d2 - 1e-3 > (double)dot(sh.p - so, sh.p - so) &&
+                            !S.mats[sh.mat].isEmissive();
+                if (!occl) {
+                    float cosL = std::max(0.f, dot(sN, Li * -1.f));
+                    float G = (cosT * cosL) / d2;
+                    float w = 1.f;
+                    if (P.mis && cosL > 1e-6f) {
+                        float pl = d2 / (cosL * S.totalArea);
+                        float pb = cosT / kPiF;
+                        w = (pl * pl) / (pl * pl + pb * pb);
+                    }
+                    for (int c = 0; c < ps.nch; c++) {
+                        float le = spectral ? lm->emissiveAt(ps.lam[c]) : lm->emissive[c];
+                        dl[c] += (alb[c] / kPiF) * le * G * S.totalArea * w;
+                    }
+                }
+            }
+            for (int c = 0; c < ps.nch; c++) rad[ps.ch0 + c] += ps.tp[c] * (dl[c] / (float)P.shadow);
+        }
+        if (P.russian && ps.bounce >= 1) {
+            float a = spectral ? alb[0] : std::max(std::max(alb[0], alb[1]), alb[2]);
+            float p = std::min(0.95f, std::max(0.05f, a));
+            if (ps.rng.next() > p) return;
+            for (int c = 0; c < ps.nch; c++) ps.tp[c] /= p;
+        }
+        float r1, r2;
+        if (P.stratified && ps.first) {
+            r1 = pr1;
+            r2 = pr2;
+        } else {
+            r1 = ps.rng.next();
+            r2 = ps.rng.next();
+        }
+        ps.first = false;
+        ps.d = fm_hemisphere(N, r1, r2);
+        ps.o = hit + N * kEps3;
+        for (int c = 0; c < ps.nch; c++) ps.tp[c] *= alb[c];
+        ps.bounce++;
+    }
+}
+
+// One sample of pixel (px, py): channel radiance and the XYZ/RGB contribution.
+void path_sample(PathCtx &C, int px, int py, int s, float rad[4], float out3[3]) {
+    const orc_path_params &P = C.P;
+    const Scene &S = C.S;
+    uint64_t k = ((uint64_t)py * (uint64_t)P.width + (uint64_t)px) * (uint64_t)P.spp + (uint64_t)s;
+    PathState ps;
+    ps.rng.state = stream_seed(P.seed, k);
+    float jx = 0.f, jy = 0.f;
+    if (P.jitter) {
+        jx = ps.rng.next() - 0.5f;
+        jy = ps.rng.next() - 0.5f;
+    }
+    float x = ((2.f * ((float)px + 0.5f + jx) / (float)P.width) - 1.f) * C.scale * C.aspect;
+    float y = -((2.f * ((float)py + 0.5f + jy) / (float)P.height) - 1.f) * C.scale;
+    ps.d = normalize(v3(x, y, -1.f));
+    ps.o = S.cam;
+    float pr1 = 0.f, pr2 = 0.f;
+    if (P.stratified) {
+        int strata = std::max(1, (int)std::round(std::sqrt((float)P.spp)));
+        int sx = s % strata, sy = (s / strata) % strata;
+        pr1 = ((float)sx + ps.rng.next()) / (float)strata;
+        pr2 = ((float)sy + ps.rng.next()) / (float)strata;
+    }
+    int N = std::clamp(P.hero, 1, 4);
+    if (P.spectral) {
+        float l0 = kLambdaMin + ps.rng.next() * (kLambdaMax - kLambdaMin);
+        float stride = (kLambdaMax - kLambdaMin) / (float)N;
+        ps.lam[0] = l0;
+        for (int c = 1; c < 4; c++) {
+            float l = l0 + stride * (float)c;
+            if (l > kLambdaMax) l -= (kLambdaMax - kLambdaMin);
+            ps.lam[c] = c < N ? l : l0;
+        }
+        ps.nch = N;
+    } else {
+        for (int c = 0; c < 4; c++) ps.lam[c] = 0.f;
+        ps.nch = 3;
+    }
+    for (int c = 0; c < 4; c++) {
+        ps.tp[c] = 1.f;
+        rad[c] = 0.f;
+    }
+    ps.ch0 = 0;
+    ps.bounce = 0;
+    ps.first = true;
+    trace_path(C, ps, pr1, pr2, rad);
+    if (P.spectral) {
+        // singleLambdaXYZ (CIE.h:140-159) per channel, averaged over N.
+        float sc = (kLambdaMax - kLambdaMin) / ybar_integral();
+        float X = 0.f, Y = 0.f, Z = 0.f;
+        for (int c = 0; c < N; c++) {
+            V3 cmf = fm_cmf(ps.lam[c]);
+            X = X + rad[c] * cmf.x * sc;
+            Y = Y + rad[c] * cmf.y * sc;
+            Z = Z + rad[c] * cmf.z * sc;
+        }
+        out3[0] = X / (float)N;
+        out3[1] = Y / (float)N;
+        out3[2] = Z / (float)N;
+    } else {
+        out3[0] = rad[0];
+        out3[1] = rad[1];
+        out3[2] = rad[2];
+    }
+}
+
+// ---------------------------------------------------------- tone mapping ----
+inline void reinhard(V3 &c) {
+    c.x = c.x / (c.x + 1.f);
+    c.y = c.y / (c.y + 1.f);
+    c.z = c.z / (c.z + 1.f);
+}
+inline void aces(V3 &c) {
+    const float A = 2.51f, B = 0.03f, Cc = 2.43f, D = 0.59f, E = 0.14f;
+    for (int i = 0; i < 3; i++) {
+        float x = c[i];
+        float v = (x * (A * x + B)) / (x * (Cc * x + D) + E);
+        if (v < 0.f) v = 0.f;
+        if (v > 1.f) v = 1.f;
+        c[i] = v;
+    }
+}
+// `(unsigned char)(255 * x + 0.5f)` as x86-64 g++ -O2 compiles it: a 32-bit
+// truncating convert, then the low byte.
+inline uint8_t quantize(float x) { return (uint8_t)(int32_t)(255 * x + 0.5f); }
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+void *orc_scene_new(void) { return new Scene(); }
+void orc_scene_free(void *s) { delete (Scene *)s; }
+
+int orc_scene_add_material(void *sp, const float albedo[3], const float emissive[3], int metallic,
+                           int transparent, float ior, float cauchyB, const float *tabA, const float *tabE) {
+    Scene &S = *(Scene *)sp;
+    Mat m;
+    m.albedo = ld3(albedo);
+    m.emissive = ld3(emissive);
+    m.metallic = metallic != 0;
+    m.transparent = transparent != 0;
+    m.ior = ior;
+    m.cauchyB = cauchyB;
+    if (tabA) {
+        std::memcpy(m.tabA, tabA, sizeof(m.tabA));
+        m.useTabA = true;
+    }
+    if (tabE) {
+        std::memcpy(m.tabE, tabE, sizeof(m.tabE));
+        m.useTabE = true;
+    }
+    S.mats.push_back(m);
+    return (int)S.mats.size() - 1;
+}
+int orc_scene_add_sphere(void *sp, const float c[3], float r, int mat) {
+    Scene &S = *(Scene *)sp;
+    S.spheres.push_back(Sph{ld3(c), r, mat});
+    return (int)S.spheres.size() - 1;
+}
+int orc_scene_add_plane(void *sp, const float o[3], const float u[3], const float v[3], int mat, int is_light) {
+    Scene &S = *(Scene *)sp;
+    Pln p = make_plane(ld3(o), ld3(u), ld3(v), mat);
+    if (is_light) {
+        Light L;
+        L.kind = LK::Plane;
+        L.plane = p;
+        L.area = p.area == 0 ? length(cross(p.u, p.v)) : p.area;  // Plane::getArea
+        S.lights.push_back(L);
+    } else {
+        S.walls.push_back(p);
+    }
+    return 0;
+}
+int orc_scene_add_triangle(void *sp, const float v[9], const float *n, int mat, int is_light) {
+    Scene &S = *(Scene *)sp;
+    Tri t = make_tri(ld3(v), ld3(v + 3), ld3(v + 6), mat);
+    if (n) {
+        t.smooth = true;
+        t.n0 = ld3(n);
+        t.n1 = ld3(n + 3);
+        t.n2 = ld3(n + 6);
+    }
+    t.orig = (int)S.tris.size();
+    if (is_light) S.lights.push_back(make_triset_light({t}));
+    S.tris.push_back(t);
+    return t.orig;
+}
+int orc_scene_add_mesh_light(void *sp, int first, int count) {
+    Scene &S = *(Scene *)sp;
+    if (first < 0 || count <= 0 || first + count > (int)S.tris.size()) return -1;
+    std::vector<Tri> set(S.tris.begin() + first, S.tris.begin() + first + count);
+    S.lights.push_back(make_triset_light(std::move(set)));
+    return 0;
+}
+void orc_scene_set_camera(void *sp, const float pos[3], float fov) {
+    Scene &S = *(Scene *)sp;
+    S.cam = ld3(pos);
+    S.fov = fov;
+}
+int orc_scene_finalize(void *sp) {
+    Scene &S = *(Scene *)sp;
+    if (S.finalized) return 0;
+    S.nodes = bvh_build(S.tris);
+    for (auto &m : S.mats) m.populate();
+    S.planes.clear();
+    for (const auto &L : S.lights)
+        if (L.kind == LK::Plane) S.planes.push_back(L.plane);
+    for (const auto &w : S.walls) S.planes.push_back(w);
+    S.totalArea = 0.f;
+    for (const auto &L : S.lights) S.totalArea += L.area;
+    S.finalized = true;
+    return 0;
+}
+int orc_scene_info_get(void *sp, orc_scene_info *o) {
+    Scene &S = *(Scene *)sp;
+    o->n_materials = (int)S.mats.size();
+    o->n_spheres = (int)S.spheres.size();
+    o->n_planes = (int)S.planes.size();
+    o->n_triangles = (int)S.tris.size();
+    o->n_nodes = (int)S.nodes.size();
+    o->n_lights = (int)S.lights.size();
+    o->total_light_area = S.totalArea;
+    o->bvh_depth = S.nodes.empty() ? 0 : bvh_depth(S.nodes, 0);
+    o->bvh_leaves = 0;
+    o->bvh_max_leaf = 0;
+    for (const auto &n : S.nodes)
+        if (n.count > 0) {
+            o->bvh_leaves++;
+            o->bvh_max_leaf = std::max(o->bvh_max_leaf, n.count);
+        }
+    return 0;
+}
+int orc_scene_get_bvh(void *sp, int32_t *nodes, int32_t *perm) {
+    Scene &S = *(Scene *)sp;
+    for (size_t i = 0; i < S.nodes.size(); i++) {
+        std::memcpy(nodes + 9 * i, S.nodes[i].bmin, 12);
+        std::memcpy(nodes + 9 * i + 3, S.nodes[i].bmax, 12);
+        nodes[9 * i + 6] = S.nodes[i].left;
+        nodes[9 * i + 7] = S.nodes[i].right;
+        nodes[9 * i + 8] = S.nodes[i].count;
+    }
+    for (size_t i = 0; i < S.tris.size(); i++) perm[i] = S.tris[i].orig;
+    return 0;
+}
+int orc_scene_get_fits(void *sp, float *fa, float *fe) {
+    Scene &S = *(Scene *)sp;
+    for (size_t i = 0; i < S.mats.size(); i++) {
+        const Mat &m = S.mats[i];
+        float a[4] = {m.fitA.c0, m.fitA.c1, m.fitA.c2, m.fitA.scale};
+        float e[4] = {m.fitE.c0, m.fitE.c1, m.fitE.c2, m.fitE.scale};
+        std::memcpy(fa + 4 * i, a, 16);
+        std::memcpy(fe + 4 * i, e, 16);
+    }
+    return 0;
+}
+int orc_scene_intersect(void *sp, const float *rays, int n, float tmax, float *t, int32_t *kind, int32_t *prim,
+                        int32_t *mat, float *hit, float *normal) {
+    Scene &S = *(Scene *)sp;
+    for (int i = 0; i < n; i++) {
+        Hit h = scene_hit(S, ld3(rays + 6 * i), ld3(rays + 6 * i + 3), tmax);
+        t[i] = h.ok ? h.t : INFINITY;
+        kind[i] = h.ok ? h.kind : -1;
+        prim[i] = h.ok ? h.prim : -1;
+        mat[i] = h.ok ? h.mat : -1;
+        if (hit) {
+            hit[3 * i] = h.ok ? h.p.x : 0.f;
+            hit[3 * i + 1] = h.ok ? h.p.y : 0.f;
+            hit[3 * i + 2] = h.ok ? h.p.z : 0.f;
+        }
+        if (normal) {
+            normal[3 * i] = h.ok ? h.N.x : 0.f;
+            normal[3 * i + 1] = h.ok ? h.N.y : 0.f;
+            normal[3 * i + 2] = h.ok ? h.N.z : 0.f;
+        }
+    }
+    return 0;
+}
+
+int orc_render_branched(void *sp, const orc_branched_params *pp, float *rgb_linear, uint8_t *rgb8) {
+    Scene &S = *(Scene *)sp;
+    const orc_branched_params &P = *pp;
+    if (!S.finalized || P.seed == 0 || S.lights.empty()) return -1;
+    Branched B{S, P, Rng{stream_seed(P.seed, 0)}};
+    V3 origin = S.cam;
+    float aspect = P.width / (float)P.height;
+    float scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * S.fov);
+    for (size_t i = 0; i < (size_t)P.height; i++) {
+        for (size_t j = 0; j < (size_t)P.width; j++) {
+            int aa = std::max(1, P.aa);
+            V3 mean, M2;
+            int taken = 0;
+            for (int s = 0; s < aa; s++) {
+                float jx = (aa > 1) ? (B.eps() - 0.5f) : 0.f;
+                float jy = (aa > 1) ? (B.eps() - 0.5f) : 0.f;
+                float x = ((2 * (j + 0.5f + jx) / (float)P.width) - 1) * scale * aspect;
+                float y = -((2 * (i + 0.5f + jy) / (float)P.height) - 1) * scale;
+                V3 dir = normalize(v3(x, y, -1.f));
+                V3 c;
+                if (P.spectral) {
+                    int N = std::clamp(P.hero, 1, 4);
+                    constexpr float kSpan = kLambdaMax - kLambdaMin;
+                    float kStride = kSpan / (float)N;
+                    Branched::SS lam;
+                    lam[0] = kLambdaMin + B.eps() * kSpan;
+                    for (int k = 1; k < N; k++) {
+                        float l = lam[0] + kStride * k;
+                        if (l > kLambdaMax) l -= kSpan;
+                        lam[k] = l;
+                    }
+                    for (int k = N; k < 4; k++) lam[k] = lam[0];
+                    Branched::SS rad = B.cast_spec(origin, dir, 0, lam);
+                    V3 xyz;
+                    for (int k = 0; k < N; k++) xyz = xyz + single_lambda_xyz(lam[k], rad[k]);
+                    c = xyz / (float)N;
+                } else {
+                    c = B.cast_rgb(origin, dir, 0);
+                }
+                taken++;
+                V3 delta = c - mean;
+                mean.x += delta.x / taken;
+                mean.y += delta.y / taken;
+                mean.z += delta.z / taken;
+                V3 delta2 = c - mean;
+                M2.x += delta.x * delta2.x;
+                M2.y += delta.y * delta2.y;
+                M2.z += delta.z * delta2.z;
+                if (P.adaptive && taken >= 4) {
+                    bool conv = true;
+                    for (int ch = 0; ch < 3; ch++) {
+                        float v = M2[ch] / (float)(taken - 1);
+                        float rel = v / (mean[ch] * mean[ch] + 0.01f);
+                        if (rel >= 0.05f) { conv = false; break; }
+                    }
+                    if (conv) break;
+                }
+            }
+            if (P.spectral) mean = xyz_to_srgb(mean);
+            size_t idx = i * P.width + j;
+            if (rgb_linear) {
+                rgb_linear[3 * idx] = mean.x;
+                rgb_linear[3 * idx + 1] = mean.y;
+                rgb_linear[3 * idx + 2] = mean.z;
+            }
+            if (rgb8) {
+                V3 c = mean;
+                if (P.aces) aces(c); else reinhard(c);
+                rgb8[3 * idx] = quantize(c.x);
+                rgb8[3 * idx + 1] = quantize(c.y);
+                rgb8[3 * idx + 2] = quantize(c.z);
+            }
+        }
+    }
+    return 0;
+}
+
+int orc_render_paths(void *sp, const orc_path_params *pp, double *accum, uint64_t *counters) {
+    Scene &S = *(Scene *)sp;
+    const orc_path_params &P = *pp;
+    if (!S.finalized || S.lights.empty()) return -1;
+    if (P.x0 < 0 || P.y0 < 0 || P.x0 + P.cw > P.width || P.y0 + P.ch > P.height) return -2;
+    int nth = P.threads > 0 ? P.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    float aspect = P.width / (float)P.height;
+    float scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * S.fov);
+    std::vector<std::array<uint64_t, 4>> cnt(nth, std::array<uint64_t, 4>{0, 0, 0, 0});
+    std::atomic<int> nextRow{0};
+    auto worker = [&](int tid) {
+        PathCtx C{S, P, scale, aspect};
+        uint64_t paths = 0;
+        for (;;) {
+            int r = nextRow.fetch_add(1);
+            if (r >= P.ch) break;
+            int py = P.y0 + r;
+            for (int cx = 0; cx < P.cw; cx++) {
+                int px = P.x0 + cx;
+                double a0 = 0, a1 = 0, a2 = 0;
+                for (int s = P.s_begin; s < P.s_end; s++) {
+                    float rad[4], o3[3];
+                    path_sample(C, px, py, s, rad, o3);
+                    a0 += (double)o3[0];
+                    a1 += (double)o3[1];
+                    a2 += (double)o3[2];
+                    paths++;
+                }
+                size_t idx = (size_t)r * P.cw + cx;
+                accum[3 * idx] += a0;
+                accum[3 * idx + 1] += a1;
+                accum[3 * idx + 2] += a2;
+            }
+        }
+        cnt[tid] = {paths, C.n_ext, C.n_shadow, C.n_emit};
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++) th.emplace_back(worker, t);
+    for (auto &t : th) t.join();
+    if (counters) {
+        for (int k = 0; k < 4; k++) {
+            uint64_t v = 0;
+            for (int t = 0; t < nth; t++) v += cnt[t][k];
+            counters[k] = v;
+        }
+    }
+    return 0;
+}
+
+int orc_trace_one(void *sp, const orc_path_params *pp, int px, int py, int sample, float rad[4], float out3[3]) {
+    Scene &S = *(Scene *)sp;
+    const orc_path_params &P = *pp;
+    float aspect = P.width / (float)P.height;
+    float scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * S.fov);
+    PathCtx C{S, P, scale, aspect};
+    path_sample(C, px, py, sample, rad, out3);
+    return 0;
+}
+
+void orc_numgen(uint64_t seed, uint64_t stream, int n, float *out) {
+    Rng r{stream_seed(seed, stream)};
+    for (int i = 0; i < n; i++) out[i] = r.next();
+}
+void orc_pcg_raw(uint64_t state, int n, uint32_t *out, uint64_t *state_out) {
+    for (int i = 0; i < n; i++) out[i] = pcg_step(state);
+    if (state_out) *state_out = state;
+}
+void orc_fit_sigmoid(const float rgb[3], float out[4]) {
+    Fit f = fit_sigmoid(ld3(rgb));
+    out[0] = f.c0; out[1] = f.c1; out[2] = f.c2; out[3] = f.scale;
+}
+float orc_eval_sigmoid(const float fit[4], float lambda) {
+    Fit f;
+    f.c0 = fit[0]; f.c1 = fit[1]; f.c2 = fit[2]; f.scale = fit[3];
+    return eval_fit(f, lambda);
+}
+float orc_ybar_integral(void) { return ybar_integral(); }
+void orc_cmf(float l, float out[3]) { out[0] = xbar(l); out[1] = ybar(l); out[2] = zbar(l); }
+void orc_single_lambda_xyz(float l, float rad, float out[3]) {
+    V3 v = single_lambda_xyz(l, rad);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+void orc_spectrum_to_xyz(const float s61[61], float out[3]) {
+    V3 v = spectrum_to_xyz(s61);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+void orc_xyz_to_srgb(const float xyz[3], float out[3]) {
+    V3 v = xyz_to_srgb(ld3(xyz));
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+void orc_srgb_to_xyz(const float rgb[3], float out[3]) {
+    V3 v = srgb_to_xyz(ld3(rgb));
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+float orc_spectrum_eval(const float s61[61], float lambda) { return spec_eval(s61, lambda); }
+void orc_dielectric(const float dir[3], const float n[3], const float hit[3], int entering, float ior, float u,
+                    float out[6]) {
+    Bounce b = dielectric(ld3(dir), ld3(n), ld3(hit), entering != 0, ior, u);
+    out[0] = b.dir.x; out[1] = b.dir.y; out[2] = b.dir.z;
+    out[3] = b.origin.x; out[4] = b.origin.y; out[5] = b.origin.z;
+}
+float orc_schlick(float c, float n1, float n2) { return schlick(c, n1, n2); }
+float orc_cauchy(float ior, float b, float l) { return cauchy_ior(ior, b, l); }
+int orc_bvh_build(const float *tris, int n, int32_t *nodes_out, int32_t *perm) {
+    std::vector<Tri> t;
+    t.reserve(n);
+    for (int i = 0; i < n; i++) {
+        Tri x = make_tri(ld3(tris + 9 * i), ld3(tris + 9 * i + 3), ld3(tris + 9 * i + 6), i);
+        x.orig = i;
+        t.push_back(x);
+    }
+    std::vector<Node> nodes = bvh_build(t);
+    if (nodes_out) {
+        for (size_t i = 0; i < nodes.size(); i++) {
+            std::memcpy(nodes_out + 9 * i, nodes[i].bmin, 12);
+            std::memcpy(nodes_out + 9 * i + 3, nodes[i].bmax, 12);
+            nodes_out[9 * i + 6] = nodes[i].left;
+            nodes_out[9 * i + 7] = nodes[i].right;
+            nodes_out[9 * i + 8] = nodes[i].count;
+        }
+    }
+    if (perm)
+        for (int i = 0; i < n; i++) perm[i] = t[i].orig;
+    return (int)nodes.size();
+}
+// SpdLoader.cpp:20-35,77-78: endpoint-clamped linear interpolation of
+// ascending (lambda, value) pairs onto the 61-sample grid.
+int orc_spd_resample(const float *lam, const float *val, int n, float out61[61]) {
+    if (n <= 0) return -1;
+    for (int i = 1; i < n; i++)
+        if (lam[i] <= lam[i - 1]) return -2;
+    for (int i = 0; i < kSamples; i++) {
+        float l = lambda_at(i);
+        float v;
+        if (l <= lam[0]) v = val[0];
+        else if (l >= lam[n - 1]) v = val[n - 1];
+        else {
+            int hi = (int)(std::upper_bound(lam, lam + n, l) - lam);
+            int lo = hi - 1;
+            float t = (l - lam[lo]) / (lam[hi] - lam[lo]);
+            v = val[lo] + t * (val[hi] - val[lo]);
+        }
+        out61[i] = v;
+    }
+    return 0;
+}
+int orc_make_icosphere(int subdiv, uint32_t seed, double radius, double amp, const double center[3],
+                       float *out) {
+    pcr::IcoParams p;
+    p.subdiv = subdiv;
+    p.seed = seed;
+    p.radius = radius;
+    p.amplitude = amp;
+    p.center[0] = center[0];
+    p.center[1] = center[1];
+    p.center[2] = center[2];
+    std::vector<float> v = pcr::make_displaced_icosphere(p);
+    if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
+    return (int)(v.size() / 9);
+}
+void orc_fm_sincos_turns(float t, float out[2]) { fm_sincos_turns(t, out[0], out[1]); }
+float orc_fm_exp(float x) { return fm_exp(x); }
+void orc_fm_cmf(float l, float out[3]) {
+    V3 v = fm_cmf(l);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+/*
+ * oracle.h -- C ABI of the CPU oracle for pcr's render path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * entry and the CPU-baseline leg of bench.py call it to compute the answer the
+ * HIP path must reproduce.  The product (pcr_amd/) never links or calls it.
+ *
+ * Two estimators live here:
+ *   - orc_render_branched: a restatement of the reference CPU renderer
+ *     (code/Renderer.cpp:113-466 render loop, :468-654 castRay,
+ *     :656-889 castRaySpectral, :891-949 sceneIntersect), including its
+ *     NumGen draw order and the mutating Vec3f::operator+ drift
+ *     (code/Includes/Vec3f.h:67-73).  Pinned byte-for-byte against the
+ *     reference's seeded golden PNGs.
+ *   - orc_render_paths: the non-branching (one path per sample) estimator of
+ *     the reference GPU backends (code/Gpu/Opengl/OpenglRenderer.cpp:783-1247),
+ *     written with per-path NumGen streams and a fixed IEEE-only math set so
+ *     the HIP wavefront tracer can be compared against it path by path.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- scene construction (mirrors code/Scenes/SceneLoader.cpp:394-516) ---- */
+void *orc_scene_new(void);
+void orc_scene_free(void *s);
+/* tab_albedo / tab_emissive: 61 samples on the 400..700 nm 5 nm grid, or NULL.
+ * Returns the material index. */
+int orc_scene_add_material(void *s, const float albedo[3], const float emissive[3],
+                           int metallic, int transparent, float ior, float cauchyB,
+                           const float *tab_albedo, const float *tab_emissive);
+int orc_scene_add_sphere(void *s, const float c[3], float r, int mat);
+/* is_light: plane goes ONLY to the light list (SceneLoader.cpp:432-447). */
+int orc_scene_add_plane(void *s, const float o[3], const float u[3], const float v[3],
+                        int mat, int is_light);
+/* v: 9 floats v0,v1,v2; n: 9 floats (smooth normals) or NULL.  is_light makes a
+ * one-triangle TriangleSet light (the triangle also stays geometry). */
+int orc_scene_add_triangle(void *s, const float v[9], const float *n, int mat, int is_light);
+/* TriangleSet light over triangles [first, first+count) in insertion order. */
+int orc_scene_add_mesh_light(void *s, int first, int count);
+void orc_scene_set_camera(void *s, const float pos[3], float fov);
+/* Builds the BVH (permuting triangles) and fits material spectra. */
+int orc_scene_finalize(void *s);
+
+typedef struct {
+    int n_materials, n_spheres, n_planes, n_triangles, n_nodes, n_lights;
+    float total_light_area;
+    int bvh_depth, bvh_leaves, bvh_max_leaf;
+} orc_scene_info;
+int orc_scene_info_get(void *s, orc_scene_info *out);
+/* nodes: n_nodes*9 int32 (6 float bit patterns, leftOrFirst, rightChild, count);
+ * perm: n_triangles, original insertion index of each triangle in leaf order. */
+int orc_scene_get_bvh(void *s, int32_t *nodes, int32_t *perm);
+/* fits: n_materials*4 each (c0,c1,c2,scale) */
+int orc_scene_get_fits(void *s, float *fit_albedo, float *fit_emissive);
+
+/* Closest hit with Renderer::sceneIntersect semantics.
+ * rays: n*6 (origin, dir).  kind: -1 miss, 0 sphere, 1 plane, 2 triangle.
+ * prim: sphere index / plane index in _planes order (lights first) /
+ * ORIGINAL triangle index.  t, matIdx, hit point, unflipped normal. */
+int orc_scene_intersect(void *s, const float *rays, int n, float tmax,
+                        float *t, int32_t *kind, int32_t *prim, int32_t *mat,
+                        float *hit, float *normal);
+
+typedef struct {
+    int width, height, depth, samples, shadow;
+    int aa;            /* aaSamples (1 = pixel centre) */
+    int hero;          /* heroSamples 1..4 */
+    int spectral, mis, russian, stratified, aces, adaptive;
+    uint64_t seed;     /* NumGen::setSeed value; must be non-zero */
+} orc_branched_params;
+/* rgb_linear: w*h*3 floats (the framebuffer before tone map);
+ * rgb8: w*h*3 bytes after tone map + quantize (Renderer.cpp:365-382). */
+int orc_render_branched(void *s, const orc_branched_params *p, float *rgb_linear,
+                        uint8_t *rgb8);
+
+typedef struct {
+    int width, height, depth, spp, shadow;
+    int jitter;        /* 1: sub-pixel jitter per path (aa > 1) */
+    int hero;          /* 1..4 */
+    int spectral, mis, russian, stratified;
+    uint64_t seed;
+    int x0, y0, cw, ch;       /* crop window in full-frame pixel coordinates */
+    int s_begin, s_end;       /* sample index range [s_begin, s_end) */
+    int threads;              /* 0 = hardware_concurrency */
+} orc_path_params;
+/* accum: cw*ch*3 doubles, SUM over samples of per-path XYZ (spectral) or RGB.
+ * counters[4]: paths, extension rays, shadow rays, emissive hits. */
+int orc_render_paths(void *s, const orc_path_params *p, double *accum, uint64_t *counters);
+
+/* ---- building blocks, exposed for known-answer tests ---- */
+void orc_numgen(uint64_t seed, uint64_t stream, int n, float *out);
+void orc_pcg_raw(uint64_t state, int n, uint32_t *out, uint64_t *state_out);
+void orc_fit_sigmoid(const float rgb[3], float out[4]);
+float orc_eval_sigmoid(const float fit[4], float lambda);
+float orc_ybar_integral(void);
+void orc_cmf(float lambda, float out[3]);                 /* glibc expf, as CIE.h */
+void orc_single_lambda_xyz(float lambda, float rad, float out[3]);
+void orc_spectrum_to_xyz(const float s61[61], float out[3]);
+void orc_xyz_to_srgb(const float xyz[3], float out[3]);
+void orc_srgb_to_xyz(const float rgb[3], float out[3]);
+float orc_spectrum_eval(const float s61[61], float lambda);
+/* dielectricBounce (Optics.h:59-82): out[6] = dir, origin */
+void orc_dielectric(const float dir[3], const float n[3], const float hit[3], int entering,
+                    float ior, float u, float out[6]);
+float orc_schlick(float cos_t, float n1, float n2);
+float orc_cauchy(float ior, float b, float lambda);
+/* Standalone BVH build over n triangles (9 floats each); returns node count. */
+int orc_bvh_build(const float *tris, int n, int32_t *nodes, int32_t *perm);
+/* SPD resampling onto the 61-sample grid (SpdLoader.cpp:20-80). */
+int orc_spd_resample(const float *lam, const float *val, int n, float out61[61]);
+/* The project's procedural mesh (pcr_amd/host/procmesh.h): returns the
+ * triangle count; out (may be NULL) receives 9 floats per triangle. */
+int orc_make_icosphere(int subdiv, uint32_t seed, double radius, double amp, const double center[3],
+                       float *out);
+/* IEEE-only math used by the non-branching estimator */
+void orc_fm_sincos_turns(float t, float out[2]);
+float orc_fm_exp(float x);
+void orc_fm_cmf(float lambda, float out[3]);
+/* Per-path estimator of one sample, for debugging: returns channel radiance
+ * (4 floats) and the XYZ/RGB (3 floats). */
+int orc_trace_one(void *s, const orc_path_params *p, int px, int py, int sample,
+                  float rad[4], float out3[3]);
+
+#ifdef __cplusplus
+}
+#endif

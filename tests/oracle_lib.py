@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+Test infrastructure: only tests/, the smoke entry and bench.py's CPU-baseline
+leg import this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle.so")
+
+_f = C.POINTER(C.c_float)
+_i32 = C.POINTER(C.c_int32)
+_u8 = C.POINTER(C.c_uint8)
+_d = C.POINTER(C.c_double)
+_u64 = C.POINTER(C.c_uint64)
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_materials", C.c_int), ("n_spheres", C.c_int), ("n_planes", C.c_int),
+                ("n_triangles", C.c_int), ("n_nodes", C.c_int), ("n_lights", C.c_int),
+                ("total_light_area", C.c_float), ("bvh_depth", C.c_int), ("bvh_leaves", C.c_int),
+                ("bvh_max_leaf", C.c_int)]
+
+
+class BranchedParams(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("depth", C.c_int), ("samples", C.c_int),
+                ("shadow", C.c_int), ("aa", C.c_int), ("hero", C.c_int), ("spectral", C.c_int),
+                ("mis", C.c_int), ("russian", C.c_int), ("stratified", C.c_int), ("aces", C.c_int),
+                ("adaptive", C.c_int), ("seed", C.c_uint64)]
+
+
+class PathParams(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("depth", C.c_int), ("spp", C.c_int),
+                ("shadow", C.c_int), ("jitter", C.c_int), ("hero", C.c_int), ("spectral", C.c_int),
+                ("mis", C.c_int), ("russian", C.c_int), ("stratified", C.c_int), ("seed", C.c_uint64),
+                ("x0", C.c_int), ("y0", C.c_int), ("cw", C.c_int), ("ch", C.c_int),
+                ("s_begin", C.c_int), ("s_end", C.c_int), ("threads", C.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_scene_new": (C.c_void_p, []),
+        "orc_scene_free": (None, [C.c_void_p]),
+        "orc_scene_add_material": (C.c_int, [C.c_void_p, _f, _f, C.c_int, C.c_int, C.c_float, C.c_float, _f, _f]),
+        "orc_scene_add_sphere": (C.c_int, [C.c_void_p, _f, C.c_float, C.c_int]),
+        "orc_scene_add_plane": (C.c_int, [C.c_void_p, _f, _f, _f, C.c_int, C.c_int]),
+        "orc_scene_add_triangle": (C.c_int, [C.c_void_p, _f, _f, C.c_int, C.c_int]),
+        "orc_scene_add_mesh_light": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "orc_scene_set_camera": (None, [C.c_void_p, _f, C.c_float]),
+        "orc_scene_finalize": (C.c_int, [C.c_void_p]),
+        "orc_scene_info_get": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
+        "orc_scene_get_bvh": (C.c_int, [C.c_void_p, _i32, _i32]),
+        "orc_scene_get_fits": (C.c_int, [C.c_void_p, _f, _f]),
+        "orc_scene_intersect": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_float, _f, _i32, _i32, _i32, _f, _f]),
+        "orc_render_branched": (C.c_int, [C.c_void_p, C.POINTER(BranchedParams), _f, _u8]),
+        "orc_render_paths": (C.c_int, [C.c_void_p, C.POINTER(PathParams), _d, _u64]),
+        "orc_trace_one": (C.c_int, [C.c_void_p, C.POINTER(PathParams), C.c_int, C.c_int, C.c_int, _f, _f]),
+        "orc_numgen": (None, [C.c_uint64, C.c_uint64, C.c_int, _f]),
+        "orc_pcg_raw": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint32), _u64]),
+        "orc_fit_sigmoid": (None, [_f, _f]),
+        "orc_eval_sigmoid": (C.c_float, [_f, C.c_float]),
+        "orc_ybar_integral": (C.c_float, []),
+        "orc_cmf": (None, [C.c_float, _f]),
+        "orc_single_lambda_xyz": (None, [C.c_float, C.c_float, _f]),
+        "orc_spectrum_to_xyz": (None, [_f, _f]),
+        "orc_xyz_to_srgb": (None, [_f, _f]),
+        "orc_srgb_to_xyz": (None, [_f, _f]),
+        "orc_spectrum_eval": (C.c_float, [_f, C.c_float]),
+        "orc_dielectric": (None, [_f, _f, _f, C.c_int, C.c_float, C.c_float, _f]),
+        "orc_schlick": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+        "orc_cauchy": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+        "orc_bvh_build": (C.c_int, [_f, C.c_int, _i32, _i32]),
+        "orc_spd_resample": (C.c_int, [_f, _f, C.c_int, _f]),
+        "orc_make_icosphere": (C.c_int, [C.c_int, C.c_uint32, C.c_double, C.c_double, _d, _f]),
+        "orc_fm_sincos_turns": (None, [C.c_float, _f]),
+        "orc_fm_exp": (C.c_float, [C.c_float]),
+        "orc_fm_cmf": (None, [C.c_float, _f]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def fp(a):
+    return a.ctypes.data_as(_f)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def numgen(seed, stream, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_numgen(seed, stream, n, fp(out))
+    return out
+
+
+def fit_sigmoid(rgb):
+    out = np.zeros(4, np.float32)
+    lib().orc_fit_sigmoid(fp(f32(rgb)), fp(out))
+    return out
+
+
+def bvh_build(tris):
+    tris = f32(tris).reshape(-1, 9)
+    n = len(tris)
+    nodes = np.zeros((max(1, 2 * n), 9), np.int32)
+    perm = np.zeros(max(1, n), np.int32)
+    nn = lib().orc_bvh_build(fp(tris), n, nodes.ctypes.data_as(_i32), perm.ctypes.data_as(_i32))
+    return nodes[:nn].copy(), perm[:n].copy()
+
+
+def make_icosphere(subdiv, seed, radius, amp, center):
+    c = np.asarray(center, np.float64)
+    n = lib().orc_make_icosphere(subdiv, seed, radius, amp, c.ctypes.data_as(_d), None)
+    out = np.zeros((n, 9), np.float32)
+    lib().orc_make_icosphere(subdiv, seed, radius, amp, c.ctypes.data_as(_d), fp(out))
+    return out
+
+
+class OracleScene:
+    """Handle to a finalized oracle scene (see tests/scene_io.py)."""
+
+    def __init__(self):
+        self.h = lib().orc_scene_new()
+        self.n_tris = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def info(self):
+        i = SceneInfo()
+        lib().orc_scene_info_get(self.h, C.byref(i))
+        return i
+
+    def bvh(self):
+        i = self.info()
+        nodes = np.zeros((max(1, i.n_nodes), 9), np.int32)
+        perm = np.zeros(max(1, i.n_triangles), np.int32)
+        lib().orc_scene_get_bvh(self.h, nodes.ctypes.data_as(_i32), perm.ctypes.data_as(_i32))
+        return nodes[:i.n_nodes], perm[:i.n_triangles]
+
+    def fits(self):
+        i = self.info()
+        a = np.zeros((i.n_materials, 4), np.float32)
+        e = np.zeros((i.n_materials, 4), np.float32)
+        lib().orc_scene_get_fits(self.h, fp(a), fp(e))
+        return a, e
+
+    def intersect(self, rays, tmax=np.finfo(np.float32).max):
+        rays = f32(rays).reshape(-1, 6)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        kind = np.zeros(n, np.int32)
+        prim = np.zeros(n, np.int32)
+        mat = np.zeros(n, np.int32)
+        hit = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        lib().orc_scene_intersect(self.h, fp(rays), n, tmax, fp(t), kind.ctypes.data_as(_i32),
+                                  prim.ctypes.data_as(_i32), mat.ctypes.data_as(_i32), fp(hit), fp(nrm))
+        return dict(t=t, kind=kind, prim=prim, mat=mat, hit=hit, normal=nrm)
+
+    def render_branched(self, width, height, depth, samples, shadow, seed, aa=1, hero=4, spectral=False,
+                        mis=False, russian=False, stratified=False, aces=False, adaptive=False):
+        p = BranchedParams(width, height, depth, samples, shadow, aa, hero, int(spectral), int(mis),
+                           int(russian), int(stratified), int(aces), int(adaptive), seed)
+        lin = np.zeros((height, width, 3), np.float32)
+        rgb8 = np.zeros((height, width, 3), np.uint8)
+        rc = lib().orc_render_branched(self.h, C.byref(p), fp(lin), rgb8.ctypes.data_as(_u8))
+        assert rc == 0, rc
+        return lin, rgb8
+
+    def render_paths(self, width, height, depth, spp, shadow, seed, jitter=False, hero=4, spectral=True,
+                     mis=False, russian=False, stratified=False, crop=None, s_range=None, threads=0):
+        x0, y0, cw, ch = crop if crop else (0, 0, width, height)
+        sb, se = s_range if s_range else (0, spp)
+        p = PathParams(width, height, depth, spp, shadow, int(jitter), hero, int(spectral), int(mis),
+                       int(russian), int(stratified), seed, x0, y0, cw, ch, sb, se, threads)
+        acc = np.zeros((ch, cw, 3), np.float64)
+        cnt = np.zeros(4, np.uint64)
+        rc = lib().orc_render_paths(self.h, C.byref(p), acc.ctypes.data_as(_d), cnt.ctypes.data_as(_u64))
+        assert rc == 0, rc
+        return acc, cnt
+
+    def trace_one(self, params, px, py, s):
+        rad = np.zeros(4, np.float32)
+        o3 = np.zeros(3, np.float32)
+        lib().orc_trace_one(self.h, C.byref(params), px, py, s, fp(rad), fp(o3))
+        return rad, o3
+
+
+def xyz_to_srgb(xyz):
+    """The reference's D65 matrix (CIE.h:115-123), applied in float64."""
+    M = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
+                  [0.0556434, -0.2040259, 1.0572252]])
+    return np.asarray(xyz, np.float64) @ M.T
