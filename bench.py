@@ -1,0 +1,149 @@
+"""Benchmark: Mpaths/s of the wavefront path tracer on the 720x720 Cornell box
+with a mirror and a Cauchy-dispersive glass sphere, depth 8, 64 samples per
+pixel, 4 shadow rays, spectral (4 hero wavelengths), FP32.
+
+One step = one full frame (720*720*64 = 33,177,600 paths) rendered on the GPU
+with the scene already resident in HBM; the frame's FP64 accumulator stays on
+the device during timing.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = dict(scene="cornell-c1", width=720, height=720, depth=8, spp=64, shadow=4, hero=4)
+# Algorithmic bytes per unit of work (minimal SoA records, each written once
+# and read once across a kernel boundary): per path 64 B (state init + pixel
+# accumulate), per extension ray 176 B (ray 56 + hit 32 + state 72 + queue
+# indices 16), per shadow ray 120 B, per emissive hit 24 B.
+BYTES_PATH, BYTES_EXT, BYTES_SHADOW, BYTES_EMIT = 64, 176, 120, 24
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    p.add_argument("--block", type=int, default=0)
+    p.add_argument("--blocks-per-cu", type=int, default=0)
+    p.add_argument("--pass-paths", type=int, default=0)
+    return p.parse_args()
+
+
+def cpu_baseline(scene_path, target_s):
+    """The CPU oracle's non-branching estimator on a bounded sample of the same
+    frame (full-width row bands, 1 sample per pixel), all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scene_io
+    osc = scene_io.build_oracle_scene(scene_io.parse_scene(scene_path))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    W, H, d, S = WORKLOAD["width"], WORKLOAD["height"], WORKLOAD["depth"], WORKLOAD["shadow"]
+    rows = 8
+    while True:
+        t0 = time.perf_counter()
+        _, cnt = osc.render_paths(W, H, d, WORKLOAD["spp"], S, 7, crop=(0, H // 2 - rows // 2, W, rows),
+                                  s_range=(0, 1), threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= target_s * 0.5 or rows >= H:
+            break
+        rows = min(H, max(rows + 1, int(rows * target_s / max(dt, 1e-3))))
+    paths = int(cnt[0])
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": "oracle non-branching estimator, %dx%d rows of the 720x720 frame, 1 spp, depth %d, "
+                      "%d shadow rays, spectral, %d paths in %.1f s" % (W, rows, d, S, paths, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every box in this pool has a single MI355X; "
+              "multi-GPU runs are not supported by this benchmark.", file=sys.stderr)
+        sys.exit(2)
+    import pcr_amd
+    scene_path = os.path.join(ROOT, "pcr_amd", "scenes", WORKLOAD["scene"] + ".json")
+    ctx = pcr_amd.Context(0)
+    cfg = {}
+    if a.block:
+        cfg["block"] = a.block
+    if a.blocks_per_cu:
+        cfg["blocks_per_cu"] = a.blocks_per_cu
+    if a.pass_paths:
+        cfg["target_pass_paths"] = a.pass_paths
+    if cfg:
+        ctx.set_config(**cfg)
+    ds = ctx.upload(pcr_amd.HostScene(scene_path))
+    kw = dict(width=WORKLOAD["width"], height=WORKLOAD["height"], depth=WORKLOAD["depth"], spp=WORKLOAD["spp"],
+              shadow=WORKLOAD["shadow"], hero=WORKLOAD["hero"], spectral=True)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+    for i in range(a.warmup):
+        ds.render(seed=1000 + i, **kw)
+    ctx.profile(enable=True, reset=True)
+    sync()
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(a.steps):
+        _, st = ds.render(seed=2000 + i, **kw)
+        stats.append(st)
+    sync()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile(enable=False, reset=True)
+    paths = sum(s.paths for s in stats)
+    ext = sum(s.ext_rays for s in stats)
+    shadow = sum(s.shadow_rays for s in stats)
+    emit = sum(s.emit_hits for s in stats)
+    alg_bytes = BYTES_PATH * paths + BYTES_EXT * ext + BYTES_SHADOW * shadow + BYTES_EMIT * emit
+    achieved = alg_bytes / (prof.bounce_ms * 1e-3) / 1e9 if prof.bounce_ms > 0 else None
+    value = paths / dt / 1e6
+    out = {
+        "metric": "Mpaths/s 720^2 Cornell d8 (mirror + Cauchy glass spheres, 64 spp, 4 shadow rays, spectral)",
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic scene (cornell-c1: 555-unit box, two r=90 spheres), per-step seeds",
+        "config": {"workload": "cornell-c1 720x720 depth 8, 64 spp, 4 shadow rays, 4 hero wavelengths",
+                   "paths_per_step": paths // a.steps, "ext_rays_per_step": ext // a.steps,
+                   "shadow_rays_per_step": shadow // a.steps, "passes_per_step": stats[0].passes,
+                   "parallelism": "1 GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_bounce (fused intersect + shade + NEE + compaction)",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "launches": prof.bounce_launches,
+            "avg_launch_ms": round(prof.bounce_ms / max(1, prof.bounce_launches), 4),
+            "algorithmic_bytes_per_launch": alg_bytes // max(1, prof.bounce_launches),
+            "queue_bytes_per_launch": prof.queue_bytes // max(1, prof.bounce_launches),
+        },
+    }
+    ds.release()
+    ctx.close()
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene_path, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

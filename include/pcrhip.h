@@ -1,0 +1,216 @@
+/*
+ * pcrhip.h -- C ABI of the MI355X (gfx950) render backend for pcr's render path.
+ *
+ * This is the device-side boundary behind pcr's renderer-backend interface
+ * (the class surface shared by Renderer, OpenglRenderer and MetalRenderer:
+ * code/Includes/Renderer.h:28-161, code/Gpu/Opengl/OpenglRenderer.h:29-117,
+ * code/Gpu/Metal/MetalRenderer.h:27-106, selected by the GpuRenderer typedef
+ * in code/Gpu/GpuRenderer.h:15-21).  The C++ class HipRenderer
+ * (pcr_amd/host/HipRenderer.h) keeps that surface and calls these functions;
+ * the host owns scene ingestion (JSON, SPD, spectral fits, BVH build), the
+ * library owns device memory, the wavefront kernels and the serving loop.
+ *
+ * Conventions: every function returns an int status (PCRHIP_OK = 0, negative
+ * on error) and records a message retrievable with pcrhip_last_error() on the
+ * calling thread.  The library allocates and frees all device memory; callers
+ * own every host buffer they pass in.  A context is thread-safe: jobs may be
+ * submitted from any thread and are executed by the context's serving thread.
+ */
+#ifndef PCRHIP_H
+#define PCRHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCRHIP_ABI_VERSION 1
+
+enum {
+    PCRHIP_OK = 0,
+    PCRHIP_EINVAL = -1,     /* bad argument */
+    PCRHIP_EDEVICE = -2,    /* HIP runtime / kernel error */
+    PCRHIP_ENOMEM = -3,     /* device memory or admission budget exhausted */
+    PCRHIP_ECANCELLED = -4, /* job cancelled before completion */
+    PCRHIP_ETIMEOUT = -5,   /* wait timed out (job still running) */
+    PCRHIP_ESTATE = -6      /* object in the wrong state (e.g. result of an unfinished job) */
+};
+
+/* Job flag bits (pcrhip_job_desc.flags); the technique toggles of the
+ * reference renderers (Renderer.h:45-112: useSpectral, useMIS, useRussian,
+ * useStratified; aaSamples > 1 enables sub-pixel jitter). */
+enum {
+    PCRHIP_SPECTRAL = 1u << 0,
+    PCRHIP_MIS = 1u << 1,
+    PCRHIP_RUSSIAN = 1u << 2,
+    PCRHIP_STRATIFIED = 1u << 3,
+    PCRHIP_JITTER = 1u << 4
+};
+
+/* Material flag bits (pcrhip_scene_desc.mat_flags), Material.h:14-62 */
+enum {
+    PCRHIP_MAT_METALLIC = 1u << 0,
+    PCRHIP_MAT_TRANSPARENT = 1u << 1,
+    PCRHIP_MAT_TAB_ALBEDO = 1u << 2,   /* 61-sample albedo table present (albedo_spd) */
+    PCRHIP_MAT_TAB_EMISSIVE = 1u << 3  /* 61-sample emission table present (emissive_spd) */
+};
+
+typedef struct pcrhip_ctx pcrhip_ctx;
+typedef struct pcrhip_scene pcrhip_scene;
+typedef struct pcrhip_job pcrhip_job;
+
+/* Last error message of the calling thread ("" if none). */
+const char *pcrhip_last_error(void);
+int pcrhip_abi_version(void);
+
+/* ------------------------------------------------------------ context ----- */
+/* Replaces the backend's GL/Metal initialisation (OpenglRenderer::initGL,
+ * OpenglRenderer.cpp:1613; MetalRenderer::initMetal, MetalRenderer.mm:3176). */
+int pcrhip_create(int device, pcrhip_ctx **out);
+int pcrhip_destroy(pcrhip_ctx *ctx);
+
+typedef struct {
+    int block;                   /* threads per block of the bounce kernel: 64, 128 or 256 */
+    int blocks_per_cu;           /* persistent grid = CUs x blocks_per_cu */
+    uint64_t target_pass_paths;  /* paths per pass (a job's samples are split into passes) */
+    uint64_t memory_budget;      /* bytes of per-job buffers the admission may hold */
+    int max_active_jobs;         /* concurrent job slots (<= 64) */
+} pcrhip_config;
+int pcrhip_get_config(pcrhip_ctx *ctx, pcrhip_config *out);
+/* Takes effect for passes started after the call. */
+int pcrhip_set_config(pcrhip_ctx *ctx, const pcrhip_config *cfg);
+
+/* -------------------------------------------------------------- scene ----- */
+/* Flat, host-resident scene in the reference's model (code/Scenes/Scene.h:22-129)
+ * after ingestion (SceneLoader.cpp:475-516): BVH already built, triangles in
+ * BVH leaf order, spectral fits computed.  Replaces the per-render upload of
+ * OpenglRenderer::uploadScene (OpenglRenderer.cpp:1657-1912). */
+typedef struct {
+    /* materials, in registry order */
+    int n_materials;
+    const float *mat_albedo;       /* n*3 linear RGB */
+    const float *mat_emissive;     /* n*3 */
+    const float *mat_fit_albedo;   /* n*4 sigmoid fit (c0, c1, c2, scale) */
+    const float *mat_fit_emissive; /* n*4 */
+    const float *mat_tab_albedo;   /* n*61, read where PCRHIP_MAT_TAB_ALBEDO is set; may be NULL */
+    const float *mat_tab_emissive; /* n*61, read where PCRHIP_MAT_TAB_EMISSIVE is set; may be NULL */
+    const uint32_t *mat_flags;     /* n */
+    const float *mat_ior;          /* n */
+    const float *mat_cauchy_b;     /* n */
+    /* spheres */
+    int n_spheres;
+    const float *spheres;          /* n*4: center xyz, radius */
+    const int32_t *sphere_mat;
+    /* planes in intersection order: light planes first, then walls (Renderer.cpp:123-130) */
+    int n_planes;
+    const float *planes;           /* n*9: origin, u, v */
+    const int32_t *plane_mat;
+    /* triangles in BVH leaf order */
+    int n_triangles;
+    const float *triangles;        /* n*9: v0, v1, v2 */
+    const float *tri_normals;      /* n*9 smooth vertex normals, or NULL */
+    const uint8_t *tri_smooth;     /* n, or NULL (all flat) */
+    const int32_t *tri_mat;
+    const int32_t *tri_orig;       /* n: index before the BVH permutation, or NULL */
+    /* BVH in the reference Node layout (Bvh.h:28-35): 9 x int32 per node
+     * (boxMin xyz, boxMax xyz as float bits, leftOrFirst, rightChild, count) */
+    int n_nodes;
+    const int32_t *nodes;
+    /* area lights in declaration order (Scene.h:41-56) */
+    int n_lights;
+    const int32_t *light_kind;     /* 0 plane, 1 triangle set */
+    const int32_t *light_first;    /* plane: index into light_planes; set: first light triangle */
+    const int32_t *light_count;    /* set: triangle count */
+    const float *light_area;       /* totalArea of each light */
+    int n_light_planes;
+    const float *light_planes;     /* n*9 */
+    const int32_t *light_plane_mat;
+    int n_light_tris;
+    const float *light_tris;       /* n*9 */
+    const int32_t *light_tri_mat;
+    const float *light_tri_cum;    /* n: cumulative area within its light */
+    /* camera (Scene.h:22-26): position, vertical fov in degrees, looks down -z */
+    float cam_pos[3];
+    float cam_fov;
+} pcrhip_scene_desc;
+
+/* Uploads the scene once; it stays resident until released. */
+int pcrhip_scene_upload(pcrhip_ctx *ctx, const pcrhip_scene_desc *desc, pcrhip_scene **out);
+int pcrhip_scene_release(pcrhip_scene *scene);
+
+typedef struct {
+    int n_inner_nodes, n_leaves, bvh_depth, traversal_stack;
+    uint64_t device_bytes;
+} pcrhip_scene_info;
+int pcrhip_scene_get_info(pcrhip_scene *scene, pcrhip_scene_info *out);
+
+/* Closest hit of n rays (host array of n*6 floats: origin, direction) with the
+ * reference's sceneIntersect semantics (Renderer.cpp:891-949, Bvh.cpp:353-444).
+ * kind: -1 miss, 0 sphere, 1 plane, 2 triangle; prim: sphere index, plane index
+ * (intersection order) or triangle index before the BVH permutation (needs
+ * tri_orig).  hit/normal (n*3, unflipped normal) may be NULL. */
+int pcrhip_intersect(pcrhip_scene *scene, const float *rays, int n, float tmax, float *t, int32_t *kind,
+                     int32_t *prim, int32_t *mat, float *hit, float *normal);
+
+/* --------------------------------------------------------------- jobs ----- */
+/* A render request: the constructor arguments and knobs of the reference
+ * renderers (Renderer.h:28-112).  spp is paths per pixel (the GPU backends'
+ * `samples`); jitter is on when flags has PCRHIP_JITTER (aaSamples > 1). */
+typedef struct {
+    int width, height, depth, spp, shadow;
+    int hero;            /* hero wavelengths 1..4 (spectral) */
+    uint32_t flags;      /* PCRHIP_* job flags */
+    uint64_t seed;       /* per-path NumGen streams derive from it */
+    int x0, y0, cw, ch;  /* crop window; cw = ch = 0 renders the full frame */
+    int priority;        /* higher is admitted first */
+} pcrhip_job_desc;
+
+/* Queues a job on the context's serving thread (continuous batching: the job
+ * shares wavefront queues with every other admitted job). */
+int pcrhip_job_submit(pcrhip_ctx *ctx, pcrhip_scene *scene, const pcrhip_job_desc *desc, pcrhip_job **out);
+/* Rows completed (scaled from completed passes), like the progressRows hook. */
+int pcrhip_job_progress(pcrhip_job *job, int *rows_done, int *rows_total);
+/* Like the cancelRequested hook: the job stops at the next pass boundary and
+ * its result reports PCRHIP_ECANCELLED. */
+int pcrhip_job_cancel(pcrhip_job *job);
+/* Waits up to timeout_ms (< 0: forever); PCRHIP_ETIMEOUT if still running. */
+int pcrhip_job_wait(pcrhip_job *job, int timeout_ms);
+
+typedef struct {
+    uint64_t paths, ext_rays, shadow_rays, emit_hits;
+    double ms_queue;   /* submit -> admitted */
+    double ms_render;  /* admitted -> finished */
+    double ms_total;   /* submit -> finished */
+    int passes;
+} pcrhip_stats;
+/* Per-pixel MEAN over spp of linear XYZ (spectral) or linear RGB, cw*ch*3
+ * floats, row-major over the crop window.  stats may be NULL. */
+int pcrhip_job_result(pcrhip_job *job, float *out, size_t n_floats, pcrhip_stats *stats);
+/* The raw FP64 per-pixel SUMS over samples (cw*ch*3 doubles). */
+int pcrhip_job_result_sums(pcrhip_job *job, double *out, size_t n_doubles);
+int pcrhip_job_release(pcrhip_job *job);
+
+/* Submit + wait + result in one call. */
+int pcrhip_render(pcrhip_ctx *ctx, pcrhip_scene *scene, const pcrhip_job_desc *desc, float *out, size_t n_floats,
+                  pcrhip_stats *stats);
+
+/* ------------------------------------------------------- measurement ----- */
+/* When enabled, the serving loop brackets every bounce-kernel launch with HIP
+ * events on its stream; the totals cover launches since the last reset. */
+int pcrhip_profile_enable(pcrhip_ctx *ctx, int enable);
+typedef struct {
+    uint64_t bounce_launches;
+    double bounce_ms;       /* summed event time of the bounce kernels */
+    uint64_t resolve_launches;
+    double resolve_ms;
+    uint64_t queue_bytes;   /* bytes of SoA queue records read + written */
+} pcrhip_profile;
+int pcrhip_profile_get(pcrhip_ctx *ctx, pcrhip_profile *out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCRHIP_H */

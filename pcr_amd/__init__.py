@@ -1,0 +1,317 @@
+"""Python binding of the MI355X render backend (ctypes over the C ABI).
+
+libpcrhip.so  device side: include/pcrhip.h (scenes, jobs, serving loop, kernels)
+libpcrhost.so host side: scene ingestion (JSON schema 1.0, SPDs, spectral fits,
+              BVH build), tone mapping and PNG output
+
+There is no fallback: if the built libraries are missing, loading raises.
+Build them with `make -C pcr_amd` (or __graft_entry__.build()).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "build")
+HIP_LIB = os.path.join(BUILD, "libpcrhip.so")
+HOST_LIB = os.path.join(BUILD, "libpcrhost.so")
+
+PCRHIP_OK, PCRHIP_EINVAL, PCRHIP_EDEVICE, PCRHIP_ENOMEM = 0, -1, -2, -3
+PCRHIP_ECANCELLED, PCRHIP_ETIMEOUT, PCRHIP_ESTATE = -4, -5, -6
+SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER = 1, 2, 4, 8, 16
+
+_f = C.POINTER(C.c_float)
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_u8 = C.POINTER(C.c_uint8)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_materials", C.c_int), ("mat_albedo", _f), ("mat_emissive", _f), ("mat_fit_albedo", _f),
+        ("mat_fit_emissive", _f), ("mat_tab_albedo", _f), ("mat_tab_emissive", _f),
+        ("mat_flags", C.POINTER(C.c_uint32)), ("mat_ior", _f), ("mat_cauchy_b", _f),
+        ("n_spheres", C.c_int), ("spheres", _f), ("sphere_mat", _i32),
+        ("n_planes", C.c_int), ("planes", _f), ("plane_mat", _i32),
+        ("n_triangles", C.c_int), ("triangles", _f), ("tri_normals", _f), ("tri_smooth", _u8),
+        ("tri_mat", _i32), ("tri_orig", _i32),
+        ("n_nodes", C.c_int), ("nodes", _i32),
+        ("n_lights", C.c_int), ("light_kind", _i32), ("light_first", _i32), ("light_count", _i32),
+        ("light_area", _f), ("n_light_planes", C.c_int), ("light_planes", _f), ("light_plane_mat", _i32),
+        ("n_light_tris", C.c_int), ("light_tris", _f), ("light_tri_mat", _i32), ("light_tri_cum", _f),
+        ("cam_pos", C.c_float * 3), ("cam_fov", C.c_float),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("target_pass_paths", C.c_uint64),
+                ("memory_budget", C.c_uint64), ("max_active_jobs", C.c_int)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_inner_nodes", C.c_int), ("n_leaves", C.c_int), ("bvh_depth", C.c_int),
+                ("traversal_stack", C.c_int), ("device_bytes", C.c_uint64)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("depth", C.c_int), ("spp", C.c_int),
+                ("shadow", C.c_int), ("hero", C.c_int), ("flags", C.c_uint32), ("seed", C.c_uint64),
+                ("x0", C.c_int), ("y0", C.c_int), ("cw", C.c_int), ("ch", C.c_int), ("priority", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("ext_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("emit_hits", C.c_uint64), ("ms_queue", C.c_double), ("ms_render", C.c_double),
+                ("ms_total", C.c_double), ("passes", C.c_int)]
+
+
+class Profile(C.Structure):
+    _fields_ = [("bounce_launches", C.c_uint64), ("bounce_ms", C.c_double), ("resolve_launches", C.c_uint64),
+                ("resolve_ms", C.c_double), ("queue_bytes", C.c_uint64)]
+
+
+# every entry point of include/pcrhip.h: name -> (restype, argtypes)
+HIP_API = {
+    "pcrhip_last_error": (C.c_char_p, []),
+    "pcrhip_abi_version": (C.c_int, []),
+    "pcrhip_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pcrhip_destroy": (C.c_int, [C.c_void_p]),
+    "pcrhip_get_config": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+    "pcrhip_set_config": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+    "pcrhip_scene_upload": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "pcrhip_scene_release": (C.c_int, [C.c_void_p]),
+    "pcrhip_scene_get_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
+    "pcrhip_intersect": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_float, _f, _i32, _i32, _i32, _f, _f]),
+    "pcrhip_job_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_void_p)]),
+    "pcrhip_job_progress": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "pcrhip_job_cancel": (C.c_int, [C.c_void_p]),
+    "pcrhip_job_wait": (C.c_int, [C.c_void_p, C.c_int]),
+    "pcrhip_job_result": (C.c_int, [C.c_void_p, _f, C.c_size_t, C.POINTER(Stats)]),
+    "pcrhip_job_result_sums": (C.c_int, [C.c_void_p, _d, C.c_size_t]),
+    "pcrhip_job_release": (C.c_int, [C.c_void_p]),
+    "pcrhip_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), _f, C.c_size_t, C.POINTER(Stats)]),
+    "pcrhip_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "pcrhip_profile_get": (C.c_int, [C.c_void_p, C.POINTER(Profile), C.c_int]),
+}
+HOST_API = {
+    "pcrhost_last_error": (C.c_char_p, []),
+    "pcrhost_scene_load": (C.c_void_p, [C.c_char_p]),
+    "pcrhost_scene_free": (None, [C.c_void_p]),
+    "pcrhost_scene_desc": (C.POINTER(SceneDesc), [C.c_void_p]),
+    "pcrhost_scene_name": (C.c_char_p, [C.c_void_p]),
+    "pcrhost_scene_bvh_depth": (C.c_int, [C.c_void_p]),
+    "pcrhost_bvh_build": (C.c_int, [_f, C.c_int, _i32, _i32]),
+    "pcrhost_fit_sigmoid": (None, [_f, _f]),
+    "pcrhost_load_spd": (C.c_int, [C.c_char_p, _f]),
+    "pcrhost_tonemap": (None, [_f, C.c_int, C.c_int, _u8]),
+    "pcrhost_write_png": (C.c_int, [C.c_char_p, _u8, C.c_int, C.c_int, C.c_char_p]),
+}
+
+_hip = None
+_host = None
+
+
+class PcrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pcrhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _bind(path, api):
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built; run `make -C pcr_amd` (the HIP path has no fallback)" % path)
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    for name, (res, args) in api.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = _bind(HIP_LIB, HIP_API)
+    return _hip
+
+
+def host():
+    global _host
+    if _host is None:
+        hip()
+        _host = _bind(HOST_LIB, HOST_API)
+    return _host
+
+
+def check(rc):
+    if rc != PCRHIP_OK:
+        raise PcrError(rc, hip().pcrhip_last_error().decode())
+    return rc
+
+
+def fp(a):
+    return a.ctypes.data_as(_f)
+
+
+class HostScene:
+    """A scene file ingested by the host library (BVH built, spectra fitted)."""
+
+    def __init__(self, path):
+        self.h = host().pcrhost_scene_load(path.encode())
+        if not self.h:
+            raise RuntimeError(host().pcrhost_last_error().decode())
+        self.desc = host().pcrhost_scene_desc(self.h).contents
+        self.name = host().pcrhost_scene_name(self.h).decode()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            host().pcrhost_scene_free(self.h)
+            self.h = None
+
+    def nodes(self):
+        d = self.desc
+        return np.ctypeslib.as_array(d.nodes, shape=(d.n_nodes * 9,)).reshape(-1, 9).copy() if d.n_nodes else \
+            np.zeros((0, 9), np.int32)
+
+    def tri_orig(self):
+        d = self.desc
+        return np.ctypeslib.as_array(d.tri_orig, shape=(d.n_triangles,)).copy() if d.n_triangles else \
+            np.zeros(0, np.int32)
+
+    def fits(self):
+        d = self.desc
+        a = np.ctypeslib.as_array(d.mat_fit_albedo, shape=(d.n_materials * 4,)).reshape(-1, 4).copy()
+        e = np.ctypeslib.as_array(d.mat_fit_emissive, shape=(d.n_materials * 4,)).reshape(-1, 4).copy()
+        return a, e
+
+
+class Context:
+    """One device context (serving thread, queues) on `device`."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(hip().pcrhip_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(hip().pcrhip_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self):
+        c = Config()
+        check(hip().pcrhip_get_config(self.h, C.byref(c)))
+        return c
+
+    def set_config(self, **kw):
+        c = self.config()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        check(hip().pcrhip_set_config(self.h, C.byref(c)))
+
+    def upload(self, host_scene):
+        s = C.c_void_p()
+        check(hip().pcrhip_scene_upload(self.h, C.byref(host_scene.desc), C.byref(s)))
+        return DeviceScene(self, s)
+
+    def profile(self, enable=None, reset=False):
+        if enable is not None:
+            check(hip().pcrhip_profile_enable(self.h, int(enable)))
+        p = Profile()
+        check(hip().pcrhip_profile_get(self.h, C.byref(p), int(reset)))
+        return p
+
+
+class DeviceScene:
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def release(self):
+        if self.h:
+            check(hip().pcrhip_scene_release(self.h))
+            self.h = None
+
+    def info(self):
+        i = SceneInfo()
+        check(hip().pcrhip_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    def intersect(self, rays, tmax=np.finfo(np.float32).max):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        kind = np.zeros(n, np.int32)
+        prim = np.zeros(n, np.int32)
+        mat = np.zeros(n, np.int32)
+        hit = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        check(hip().pcrhip_intersect(self.h, fp(rays), n, float(tmax), fp(t), kind.ctypes.data_as(_i32),
+                                     prim.ctypes.data_as(_i32), mat.ctypes.data_as(_i32), fp(hit), fp(nrm)))
+        return dict(t=t, kind=kind, prim=prim, mat=mat, hit=hit, normal=nrm)
+
+    def submit(self, **kw):
+        return Job(self, job_desc(**kw))
+
+    def render(self, **kw):
+        j = self.submit(**kw)
+        try:
+            j.wait()
+            return j.result()
+        finally:
+            j.release()
+
+
+def job_desc(width, height, depth, spp, shadow, seed, spectral=True, hero=4, mis=False, russian=False,
+             stratified=False, jitter=False, crop=None, priority=0):
+    flags = (SPECTRAL if spectral else 0) | (MIS if mis else 0) | (RUSSIAN if russian else 0) | \
+            (STRATIFIED if stratified else 0) | (JITTER if jitter else 0)
+    x0, y0, cw, ch = crop if crop else (0, 0, 0, 0)
+    return JobDesc(width, height, depth, spp, shadow, hero, flags, seed, x0, y0, cw, ch, priority)
+
+
+class Job:
+    def __init__(self, scene, desc):
+        self.desc = desc
+        self.cw = desc.cw or desc.width
+        self.ch = desc.ch or desc.height
+        h = C.c_void_p()
+        check(hip().pcrhip_job_submit(scene.ctx.h, scene.h, C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def wait(self, timeout_ms=-1):
+        rc = hip().pcrhip_job_wait(self.h, timeout_ms)
+        if rc == PCRHIP_ETIMEOUT:
+            return False
+        check(rc)
+        return True
+
+    def progress(self):
+        a, b = C.c_int(), C.c_int()
+        check(hip().pcrhip_job_progress(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def cancel(self):
+        check(hip().pcrhip_job_cancel(self.h))
+
+    def result(self):
+        out = np.zeros((self.ch, self.cw, 3), np.float32)
+        st = Stats()
+        check(hip().pcrhip_job_result(self.h, fp(out), out.size, C.byref(st)))
+        return out, st
+
+    def sums(self):
+        out = np.zeros((self.ch, self.cw, 3), np.float64)
+        check(hip().pcrhip_job_result_sums(self.h, out.ctypes.data_as(_d), out.size))
+        return out
+
+    def release(self):
+        if self.h:
+            check(hip().pcrhip_job_release(self.h))
+            self.h = None

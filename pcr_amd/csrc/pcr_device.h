@@ -1,0 +1,416 @@
+// pcr_device.h -- device-side data layout and arithmetic of the HIP path tracer.
+//
+// Compiled for gfx950 with -ffp-contract=off and correctly rounded division /
+// square root, so every + - * / sqrt below rounds exactly like the reference's
+// x86-64 build.  Operation order follows the reference line by line where it
+// fixes results:
+//   ray/triangle   code/Includes/Triangle.h:53-92
+//   ray/sphere     code/Includes/Sphere.h:20-39
+//   ray/plane      code/Includes/Plane.h:33-54
+//   ray/box, BVH   code/Bvh/Bvh.cpp:294-311, 353-444
+//   scene order    code/Renderer.cpp:891-949 (spheres, planes, then BVH)
+//   dielectric     code/Includes/Optics.h:22-82
+//   spectra        code/Includes/Spectrum.h:82-90, RGBToSpectrum.h:122-129,
+//                  Material.h:92-101
+//   NumGen         code/NumGen.cpp:24-80 (one PCG32 stream per path)
+// Transcendentals the reference takes from libm (cosf/sinf for the
+// cosine-hemisphere sample, expf for the CIE fit, powf for Schlick) are
+// replaced by fixed + - * / polynomials defined here, so the CPU oracle can
+// evaluate the same function bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcr {
+
+// ------------------------------------------------------------------ flags --
+enum : uint32_t {
+    JF_SPECTRAL = 1u << 0,
+    JF_MIS = 1u << 1,
+    JF_RUSSIAN = 1u << 2,
+    JF_STRATIFIED = 1u << 3,
+    JF_JITTER = 1u << 4,
+    JF_GUIDED = 1u << 5,
+};
+enum : uint32_t { MF_METALLIC = 1u, MF_TRANSPARENT = 2u, MF_TAB_ALBEDO = 4u, MF_TAB_EMISSIVE = 8u, MF_EMISSIVE = 16u };
+
+// -------------------------------------------------------- HBM scene layout --
+// Material: 96 bytes.
+struct DMat {
+    float4 albedo;    // rgb, w: ior
+    float4 emissive;  // rgb, w: cauchyB
+    float4 fitA;      // c0 c1 c2 scale
+    float4 fitE;
+    uint32_t flags;
+    int32_t tab;      // offset (floats) of the 2x61 table block, or -1
+    int32_t pad0, pad1;
+    float4 pad2;
+};
+// Area light: 80 bytes.  Plane: o,u,v,N of the parallelogram.  TriangleSet:
+// [first, first+count) into the light-triangle array.
+struct DLight {
+    int32_t kind;     // 0 plane, 1 triangle set
+    int32_t first;
+    int32_t count;
+    float area;
+    float4 o;         // xyz, w: material (bits)
+    float4 u;
+    float4 v;
+    float4 n;
+};
+
+struct DScene {
+    const float4 *sph;      // center.xyz, radius
+    const int32_t *sph_mat;
+    const float4 *pl;       // 4 x float4: o.xyz|uu, u.xyz|vv, v.xyz|mat, N.xyz|0
+    const float4 *tri;      // 3 x float4: v0.xyz|mat, e1.xyz|smooth, e2.xyz|0   (leaf order)
+    const float4 *tri_n;    // 4 x float4: n0, n1, n2, flatN
+    const float4 *bvh;      // 4 x float4 per inner node: child boxes + refs
+    const int2 *leaves;     // (first, count)
+    const DMat *mat;
+    const float *tabs;      // spectral tables, 61 floats each
+    const DLight *lights;
+    const float4 *ltri;     // 4 x float4: v0|mat, e1|cum, e2|0, flatN|0
+    int32_t n_sph, n_pl, n_tri, n_inner, n_leaves, n_mat, n_lights;
+    int32_t root_ref;       // >= 0 inner node, < 0 leaf ~index; valid if has_bvh
+    int32_t has_bvh;
+    float total_area;
+    float4 root_lo, root_hi;
+    float3 cam;
+    float fov;
+};
+
+// ------------------------------------------------------------- vec math ----
+// Plain operators: with -ffp-contract=off nothing is fused, and with
+// -fhip-fp32-correctly-rounded-divide-sqrt division and sqrtf round
+// correctly.  (The __f*_rn intrinsics are avoided on purpose: without
+// OCML_BASIC_ROUNDED_OPERATIONS __fsqrt_rn is the native approximation.)
+__device__ __forceinline__ float fadd(float a, float b) { return a + b; }
+__device__ __forceinline__ float fsub(float a, float b) { return a - b; }
+__device__ __forceinline__ float fmul(float a, float b) { return a * b; }
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float a, float b, float c) { return V3{a, b, c}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return v3(fadd(a.x, b.x), fadd(a.y, b.y), fadd(a.z, b.z)); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return v3(fsub(a.x, b.x), fsub(a.y, b.y), fsub(a.z, b.z)); }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return v3(fmul(a.x, s), fmul(a.y, s), fmul(a.z, s)); }
+__device__ __forceinline__ float dot(V3 a, V3 b) {
+    return fadd(fadd(fmul(a.x, b.x), fmul(a.y, b.y)), fmul(a.z, b.z));
+}
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(fsub(fmul(a.y, b.z), fmul(a.z, b.y)), fsub(fmul(a.z, b.x), fmul(a.x, b.z)),
+              fsub(fmul(a.x, b.y), fmul(a.y, b.x)));
+}
+__device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
+__device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
+__device__ __forceinline__ V3 normalize(V3 a) {
+    float m = sqrt_rn(dot(a, a));
+    return v3(div_rn(a.x, m), div_rn(a.y, m), div_rn(a.z, m));
+}
+__device__ __forceinline__ V3 xyz(float4 f) { return v3(f.x, f.y, f.z); }
+// std::max / std::min semantics (first argument wins on ties and NaN)
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+
+constexpr float kPiF = 3.14159274101257324f;
+constexpr float kEps3 = 0.00100000004749745131f;  // (float)1e-3
+constexpr float kYbarIntegral = 106.895210266113281f;  // CIE::yBarIntegral() (CIE.h:79-88)
+
+// ------------------------------------------------------------ NumGen -------
+__device__ __forceinline__ uint64_t stream_seed(uint64_t seed, uint64_t k) {
+    uint64_t s = seed ^ (k * 6364136223846793005ULL + 1442695040888963407ULL);
+    return s == 0 ? 1ULL : s;
+}
+__device__ __forceinline__ float rng_next(uint64_t &state) {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + 1442695040888963407ULL;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    uint32_t r = (xs >> rot) | (xs << ((32u - rot) & 31u));
+    return fmul((float)r, 2.3283064365386963e-10f);
+}
+
+// --------------------------------------------------- IEEE-only math set ----
+__device__ __forceinline__ void fm_sincos_turns(float t, float &s, float &c) {
+    float q = floorf(fadd(fmul(t, 4.f), 0.5f));
+    int n = (int)q;
+    float r = fsub(t, fmul(q, 0.25f));
+    float a = fmul(r, 6.28318548f);
+    float a2 = fmul(a, a);
+    float ps = fadd(-1.f / 6.f, fmul(a2, fadd(1.f / 120.f, fmul(a2, fadd(-1.f / 5040.f,
+                                                                   fmul(a2, 1.f / 362880.f))))));
+    float sn = fadd(a, fmul(a, fmul(a2, ps)));
+    float pc = fadd(-0.5f, fmul(a2, fadd(1.f / 24.f, fmul(a2, fadd(-1.f / 720.f,
+                 fmul(a2, fadd(1.f / 40320.f, fmul(a2, -1.f / 3628800.f))))))));
+    float cs = fadd(1.f, fmul(a2, pc));
+    switch (n & 3) {
+    case 0: s = sn; c = cs; break;
+    case 1: s = cs; c = -sn; break;
+    case 2: s = -sn; c = -cs; break;
+    default: s = -cs; c = sn; break;
+    }
+}
+__device__ __forceinline__ float fm_exp(float x) {
+    if (x < -103.f) return 0.f;
+    if (x > 88.f) return __builtin_huge_valf();
+    float kf = floorf(fadd(fmul(x, 1.44269504f), 0.5f));
+    float r = fsub(x, fmul(kf, 0.693145751953125f));
+    r = fsub(r, fmul(kf, 1.42860677e-06f));
+    float p = fadd(1.f / 720.f, fmul(r, 1.f / 5040.f));
+    p = fadd(1.f / 120.f, fmul(r, p));
+    p = fadd(1.f / 24.f, fmul(r, p));
+    p = fadd(1.f / 6.f, fmul(r, p));
+    p = fadd(0.5f, fmul(r, p));
+    p = fadd(1.f, fmul(r, p));
+    p = fadd(1.f, fmul(r, p));
+    int k = (int)kf;
+    if (k >= -126) return fmul(p, __uint_as_float((uint32_t)(k + 127) << 23));
+    float s1 = __uint_as_float((uint32_t)(k + 64 + 127) << 23);
+    float s2 = __uint_as_float((uint32_t)(-64 + 127) << 23);
+    return fmul(fmul(p, s1), s2);
+}
+__device__ __forceinline__ float fm_wyman(float l, float mu, float s1, float s2) {
+    float t = (l < mu) ? fmul(fsub(l, mu), s1) : fmul(fsub(l, mu), s2);
+    return fm_exp(fmul(fmul(-0.5f, t), t));
+}
+__device__ __forceinline__ V3 fm_cmf(float l) {
+    float xb = fsub(fadd(fmul(0.362f, fm_wyman(l, 442.0f, 0.0624f, 0.0374f)),
+                                   fmul(1.056f, fm_wyman(l, 599.8f, 0.0264f, 0.0323f))),
+                         fmul(0.065f, fm_wyman(l, 501.1f, 0.0490f, 0.0382f)));
+    float yb = fadd(fmul(0.821f, fm_wyman(l, 568.8f, 0.0213f, 0.0247f)),
+                         fmul(0.286f, fm_wyman(l, 530.9f, 0.0613f, 0.0322f)));
+    float zb = fadd(fmul(1.217f, fm_wyman(l, 437.0f, 0.0845f, 0.0278f)),
+                         fmul(0.681f, fm_wyman(l, 459.0f, 0.0385f, 0.0725f)));
+    return v3(xb, yb, zb);
+}
+
+// ------------------------------------------------------------- spectra -----
+__device__ __forceinline__ float spec_eval(const float *s, float l) {
+    if (l < 400.f || l > 700.f) return 0.f;
+    float t = div_rn(fsub(l, 400.f), 5.f);
+    int i = (int)t;
+    if (i >= 60) return s[60];
+    float f = fsub(t, (float)i);
+    return fadd(fmul(s[i], fsub(1.f, f)), fmul(s[i + 1], f));
+}
+__device__ __forceinline__ float sigmoid(float x) {
+    return fadd(0.5f, div_rn(x, fmul(2.f, sqrt_rn(fadd(1.f, fmul(x, x))))));
+}
+__device__ __forceinline__ float eval_fit(float4 f, float l) {
+    float L = div_rn(fsub(l, 550.f), 150.f);
+    float p = fadd(f.x, fmul(L, fadd(f.y, fmul(L, f.z))));
+    return fmul(sigmoid(p), f.w);
+}
+__device__ __forceinline__ float albedo_at(const DMat &m, const float *tabs, float l) {
+    if (m.flags & MF_TAB_ALBEDO) return spec_eval(tabs + m.tab, l);
+    return smin(eval_fit(m.fitA, l), 1.f);
+}
+__device__ __forceinline__ float emissive_at(const DMat &m, const float *tabs, float l) {
+    if (m.flags & MF_TAB_EMISSIVE) return spec_eval(tabs + m.tab + 61, l);
+    return eval_fit(m.fitE, l);
+}
+__device__ __forceinline__ float cauchy_ior(float ior, float B, float l) {
+    return fadd(ior, div_rn(fmul(B, 1e4f), fmul(l, l)));
+}
+
+// ------------------------------------------------------- intersections -----
+// Moller-Trumbore; on success updates closest and the (triangle, u, v) record.
+__device__ __forceinline__ bool tri_test(const float4 *__restrict__ tri, int ti, V3 o, V3 d, float &closest,
+                                         int &best, float &bu, float &bv) {
+    float4 a = tri[3 * ti], b = tri[3 * ti + 1], c = tri[3 * ti + 2];
+    V3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
+    V3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (fabsf(det) < 1e-6f) return false;
+    float inv = div_rn(1.f, det);
+    V3 tv = sub(o, v0);
+    float u = fmul(dot(tv, pv), inv);
+    if (u < 0.f || u > 1.f) return false;
+    V3 qv = cross(tv, e1);
+    float v = fmul(dot(d, qv), inv);
+    if (v < 0.f || fadd(u, v) > 1.f) return false;
+    float t = fmul(dot(e2, qv), inv);
+    if (t <= 1e-6f || t >= closest) return false;
+    closest = t;
+    best = ti;
+    bu = u;
+    bv = v;
+    return true;
+}
+// Slab test against [lo, hi] for the segment (0, seg); inv = 1/d per axis.
+__device__ __forceinline__ bool box_test(V3 o, V3 inv, float lx, float ly, float lz, float hx, float hy, float hz,
+                                         float seg, float &tnear) {
+    float tmin = 0.f, tmax = seg;
+    float t1 = fmul(fsub(lx, o.x), inv.x), t2 = fmul(fsub(hx, o.x), inv.x);
+    if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
+    tmin = smax(tmin, t1);
+    tmax = smin(tmax, t2);
+    if (tmin > tmax) return false;
+    t1 = fmul(fsub(ly, o.y), inv.y);
+    t2 = fmul(fsub(hy, o.y), inv.y);
+    if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
+    tmin = smax(tmin, t1);
+    tmax = smin(tmax, t2);
+    if (tmin > tmax) return false;
+    t1 = fmul(fsub(lz, o.z), inv.z);
+    t2 = fmul(fsub(hz, o.z), inv.z);
+    if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
+    tmin = smax(tmin, t1);
+    tmax = smin(tmax, t2);
+    if (tmin > tmax) return false;
+    tnear = tmin;
+    return true;
+}
+
+struct HitRec {
+    float t;
+    int kind;   // -1 miss, 0 sphere, 1 plane, 2 triangle
+    int prim;   // sphere / plane / leaf-order triangle index
+    float u, v; // barycentrics of a triangle hit
+};
+
+// Per-thread traversal stack in LDS, entry-major so a wave's 64 lanes touch 64
+// consecutive dwords: idx[k * BLOCK + lane].
+struct LdsStack {
+    int32_t *idx;
+    float *tn;
+    int stride;
+};
+
+// Renderer::sceneIntersect semantics with the BVH traversal of Bvh::intersect.
+// The reference pushes both children (far first) and pops the near one; here
+// the near child is taken directly and only the far one is pushed, which is the
+// same visiting order.  Pushes that the reference's 64-entry stack would drop
+// are dropped here too (never reached for trees shallower than 63).
+template <int STACK>
+__device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, float tmax, const LdsStack &st) {
+    HitRec h;
+    h.kind = -1;
+    h.prim = -1;
+    h.u = h.v = 0.f;
+    float closest = tmax;
+    for (int i = 0; i < S.n_sph; i++) {
+        float4 s = S.sph[i];
+        V3 cp = sub(xyz(s), o);
+        float rl = dot(d, cp);
+        float tSq = fsub(dot(cp, cp), fmul(rl, rl));
+        float r2 = fmul(s.w, s.w);
+        if (tSq > r2) continue;
+        float td = sqrt_rn(fsub(r2, tSq));
+        float t0 = fsub(rl, td);
+        float t1 = fadd(rl, td);
+        if (t0 < 0) t0 = t1;
+        if (t0 < 0) continue;
+        if (t0 >= closest) continue;
+        closest = t0;
+        h.kind = 0;
+        h.prim = i;
+    }
+    for (int i = 0; i < S.n_pl; i++) {
+        float4 po = S.pl[4 * i], pu = S.pl[4 * i + 1], pv = S.pl[4 * i + 2], pn = S.pl[4 * i + 3];
+        V3 N = xyz(pn);
+        float denom = dot(d, N);
+        if (fabsf(denom) <= 1e-6f) continue;
+        float t0 = div_rn(dot(sub(xyz(po), o), N), denom);
+        if (t0 <= 1e-6f || t0 > closest) continue;
+        V3 th = add(o, mul(d, t0));
+        V3 lh = sub(th, xyz(po));
+        float s = div_rn(dot(lh, xyz(pu)), po.w);
+        float q = div_rn(dot(lh, xyz(pv)), pu.w);
+        if (s > 1 || s < 0 || q > 1 || q < 0) continue;
+        if (t0 >= closest) continue;
+        closest = t0;
+        h.kind = 1;
+        h.prim = i;
+    }
+    if (S.has_bvh) {
+        V3 inv = v3(div_rn(1.f, d.x), div_rn(1.f, d.y), div_rn(1.f, d.z));
+        float tr;
+        if (box_test(o, inv, S.root_lo.x, S.root_lo.y, S.root_lo.z, S.root_hi.x, S.root_hi.y, S.root_hi.z, closest,
+                     tr)) {
+            int best = -1;
+            float bu = 0.f, bv = 0.f;
+            int top = 0;  // entries below the current node
+            int cur = S.root_ref;
+            for (;;) {
+                if (cur < 0) {
+                    int2 lf = S.leaves[~cur];
+                    for (int k = 0; k < lf.y; k++) tri_test(S.tri, lf.x + k, o, d, closest, best, bu, bv);
+                } else {
+                    const float4 *nd = S.bvh + 4 * cur;
+                    float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+                    float tL, tR;
+                    bool hL = box_test(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, closest, tL);
+                    bool hR = box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, closest, tR);
+                    int rl = __float_as_int(e.x), rr = __float_as_int(e.y);
+                    if (hL && hR) {
+                        // reference: `if (top + 2 > 64) continue;` with `top` counting
+                        // the entries below the current node
+                        if (top + 2 <= 64 && top + 1 <= STACK) {
+                            bool leftNear = tL <= tR;
+                            st.idx[top * st.stride] = leftNear ? rr : rl;
+                            st.tn[top * st.stride] = leftNear ? tR : tL;
+                            top++;
+                            cur = leftNear ? rl : rr;
+                            continue;
+                        }
+                    } else if (hL) {
+                        if (top + 1 <= 64) { cur = rl; continue; }
+                    } else if (hR) {
+                        if (top + 1 <= 64) { cur = rr; continue; }
+                    }
+                }
+                // pop: drop entries whose entry distance is beyond the closest hit
+                bool found = false;
+                while (top > 0) {
+                    top--;
+                    float tn = st.tn[top * st.stride];
+                    if (tn > closest) continue;
+                    cur = st.idx[top * st.stride];
+                    found = true;
+                    break;
+                }
+                if (!found) break;
+            }
+            if (best >= 0) {
+                h.kind = 2;
+                h.prim = best;
+                h.u = bu;
+                h.v = bv;
+            }
+        }
+    }
+    h.t = closest;
+    return h;
+}
+
+// Hit point and unflipped normal for a hit record (Renderer.cpp:905-906,
+// Plane.h:259, Triangle.h:343-357): hit = o + d * t for every primitive kind.
+__device__ __forceinline__ void hit_geometry(const DScene &S, const HitRec &h, V3 o, V3 d, V3 &P, V3 &N, int &mat) {
+    P = add(o, mul(d, h.t));
+    if (h.kind == 0) {
+        N = normalize(sub(P, xyz(S.sph[h.prim])));
+        mat = S.sph_mat[h.prim];
+    } else if (h.kind == 1) {
+        N = xyz(S.pl[4 * h.prim + 3]);
+        mat = __float_as_int(S.pl[4 * h.prim + 2].w);
+    } else {
+        float4 a = S.tri[3 * h.prim], b = S.tri[3 * h.prim + 1];
+        mat = __float_as_int(a.w);
+        const float4 *tn = S.tri_n + 4 * h.prim;
+        if (__float_as_int(b.w)) {
+            float w = fsub(fsub(1.f, h.u), h.v);
+            float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
+            V3 in = v3(fadd(fadd(fmul(n0.x, w), fmul(n1.x, h.u)), fmul(n2.x, h.v)),
+                       fadd(fadd(fmul(n0.y, w), fmul(n1.y, h.u)), fmul(n2.y, h.v)),
+                       fadd(fadd(fmul(n0.z, w), fmul(n1.z, h.u)), fmul(n2.z, h.v)));
+            N = normalize(in);
+        } else {
+            N = xyz(tn[3]);
+        }
+    }
+}
+
+}  // namespace pcr

@@ -1,0 +1,92 @@
+// pcr_kernels.h -- job descriptors and kernel argument blocks shared by the
+// host-side driver (pcrhip.cpp) and the kernels (wavefront.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pcr_device.h"
+
+namespace pcr {
+
+// One render job's view of the current pass.  Lives in a device-resident job
+// table indexed by the slot id carried in every path's state, so paths of
+// different jobs (different scenes, resolutions, depths) share one queue.
+struct DJob {
+    DScene S;
+    int32_t W, H;            // full frame
+    int32_t x0, y0, cw, ch;  // crop window rendered by this job
+    int32_t depth, shadow, spp, hero;
+    uint32_t flags;
+    int32_t strata;          // round(sqrt(spp)) when stratified
+    uint64_t seed;
+    float scale, aspect;     // tan(fov/2) (host libm) and W/H
+    int32_t spp_pass;        // samples per pixel in one pass
+    int32_t s_begin;         // first sample index of the current pass
+    float4 *radiance;        // [cw*ch*spp_pass] channel radiance of each path
+    double *accum;           // [cw*ch*3] running sums of XYZ (or RGB)
+};
+
+// Structure-of-arrays ray queue (80 bytes per ray).
+struct QueueView {
+    float4 *od0;   // origin.xyz, hero wavelength
+    float4 *od1;   // direction.xyz, packed state word
+    float4 *tp;    // throughput per channel
+    float4 *rad;   // radiance gathered so far per channel
+    uint4 *rng;    // PCG32 state lo/hi, path index, spare
+};
+
+// Generation segment: queue items [begin, begin+count) are paths
+// [path0, path0+count) of job slot `slot`.
+struct GenSeg {
+    int32_t slot;
+    uint32_t begin, count, path0;
+};
+struct ResolveSeg {
+    int32_t slot;
+    uint32_t begin, count;
+    uint32_t pad;
+};
+
+constexpr int kMaxSegs = 64;
+
+struct BounceArgs {
+    const DJob *jobs;
+    const GenSeg *segs;
+    int32_t n_segs;
+    uint32_t n_gen;               // generating bounce: number of work items
+    QueueView in, out;
+    const uint32_t *count_in;     // live rays in `in`
+    uint32_t *count_out;          // rays appended to `out`
+    uint32_t *work;               // work-stealing counter of this launch
+    uint32_t capacity;            // rays `out` can hold
+    uint32_t *overflow;           // set if an append would exceed capacity
+    unsigned long long *counters; // paths, extension rays, shadow rays, emissive hits
+};
+
+struct ResolveArgs {
+    const DJob *jobs;
+    const ResolveSeg *segs;
+    int32_t n_segs;
+};
+
+struct IntersectArgs {
+    DScene S;
+    const float *rays;
+    uint32_t n;
+    float tmax;
+    const int32_t *tri_orig;  // leaf-order -> original triangle index (may be null)
+    float *t;
+    int32_t *kind, *prim, *mat;
+    float *hit, *normal;
+};
+
+}  // namespace pcr
+
+extern "C" {
+hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block, hipStream_t s);
+hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, uint32_t n_items, hipStream_t s);
+hipError_t pcr_launch_intersect(const pcr::IntersectArgs *a, int stack, hipStream_t s);
+hipError_t pcr_launch_reset(uint32_t *p, int n, hipStream_t s);
+hipError_t pcr_launch_advance(pcr::DJob *jobs, const int32_t *slots, int n, hipStream_t s);
+}

@@ -1,0 +1,957 @@
+// pcrhip.cpp -- the C ABI of include/pcrhip.h: device scenes, the serving
+// loop (continuous batching of render jobs into shared wavefront queues) and
+// the pass driver that launches the kernels of wavefront.hip.
+//
+// Serving model.  Each admitted job owns a slot of the device job table, an FP64
+// accumulator and a per-pass radiance buffer.  A pass takes every active job's
+// next batch of samples, generates all their paths into one queue, runs the
+// bounce kernels up to the deepest job's depth and resolves every job's pixels.
+// Jobs arriving while a pass runs are admitted at the next pass boundary as
+// slots and the memory budget allow; finished jobs leave at pass boundaries.
+// A path's samples, arithmetic and per-pixel accumulation order do not depend
+// on which other jobs share its pass, so a job's result is the same alone and
+// in any mix.
+#include "../../include/pcrhip.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <numbers>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "pcr_kernels.h"
+
+using namespace pcr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(PCRHIP_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+using Clock = std::chrono::steady_clock;
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// host-side vector math with the reference's operation order (Vec3f.h)
+struct H3 {
+    float x, y, z;
+};
+H3 h3(const float *p) { return H3{p[0], p[1], p[2]}; }
+H3 hsub(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+float hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+H3 hcross(H3 a, H3 b) { return H3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+H3 hnorm(H3 a) {
+    float m = std::sqrt(hdot(a, a));
+    return H3{a.x / m, a.y / m, a.z / m};
+}
+float4 f4(H3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+float ibits(int32_t v) {
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+
+constexpr int kSlots = 64;
+constexpr int kMaxDepthBounces = 255;
+
+}  // namespace
+
+struct pcrhip_scene {
+    pcrhip_ctx *ctx = nullptr;
+    DScene S{};
+    std::vector<void *> allocs;
+    int32_t *tri_orig = nullptr;
+    int stack = 8;
+    int depth = 0;
+    int n_inner = 0, n_leaves = 0;
+    uint64_t bytes = 0;
+    std::atomic<int> jobs{0};
+};
+
+struct pcrhip_job {
+    pcrhip_ctx *ctx = nullptr;
+    pcrhip_scene *scene = nullptr;
+    pcrhip_job_desc d{};
+    int cw = 0, ch = 0;
+    enum State { QUEUED, RUNNING, DONE } state = QUEUED;
+    int status = PCRHIP_OK;
+    std::string error;
+    std::atomic<bool> cancel{false};
+    int slot = -1;
+    int s_next = 0;        // first sample of the next pass
+    int spp_pass = 1;      // samples in the current pass
+    int passes = 0;
+    double *accum = nullptr;
+    float4 *radiance = nullptr;
+    uint64_t rad_paths = 0;  // capacity of `radiance` in paths
+    uint64_t bytes = 0;
+    std::vector<double> sums;
+    uint64_t counters[4] = {0, 0, 0, 0};
+    Clock::time_point t_submit, t_admit, t_done;
+    bool released = false;
+};
+
+struct pcrhip_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    pcrhip_config cfg{};
+    // device state
+    DJob *jobs_dev = nullptr;
+    GenSeg *segs_dev = nullptr;
+    ResolveSeg *rsegs_dev = nullptr;
+    uint32_t *ctrs = nullptr;  // [2 * kMaxDepthBounces + 2]: per bounce (count_out, work), overflow
+    unsigned long long *slot_counters = nullptr;  // [kSlots * 4]
+    QueueView qa{}, qb{};
+    uint64_t qcap = 0;
+    DJob *jobs_host = nullptr;  // pinned staging of the job table
+    // serving
+    std::mutex m;
+    std::condition_variable cv;       // worker wake-up
+    std::condition_variable done_cv;  // job completion
+    std::thread worker;
+    bool stop = false;
+    std::deque<pcrhip_job *> pending;
+    pcrhip_job *active[kSlots] = {};
+    uint64_t mem_in_use = 0;
+    std::string fatal;
+    // profiling
+    std::atomic<bool> profile{false};
+    std::mutex pm;
+    pcrhip_profile prof{};
+};
+
+namespace {
+
+void free_queue(QueueView &q) {
+    (void)hipFree(q.od0);
+    (void)hipFree(q.od1);
+    (void)hipFree(q.tp);
+    (void)hipFree(q.rad);
+    (void)hipFree(q.rng);
+    q = QueueView{};
+}
+hipError_t alloc_queue(QueueView &q, uint64_t n) {
+    hipError_t e;
+    if ((e = hipMalloc(&q.od0, n * sizeof(float4))) != hipSuccess) return e;
+    if ((e = hipMalloc(&q.od1, n * sizeof(float4))) != hipSuccess) return e;
+    if ((e = hipMalloc(&q.tp, n * sizeof(float4))) != hipSuccess) return e;
+    if ((e = hipMalloc(&q.rad, n * sizeof(float4))) != hipSuccess) return e;
+    return hipMalloc(&q.rng, n * sizeof(uint4));
+}
+
+int ensure_queues(pcrhip_ctx *c, uint64_t need) {
+    if (need <= c->qcap) return PCRHIP_OK;
+    free_queue(c->qa);
+    free_queue(c->qb);
+    c->qcap = 0;
+    uint64_t cap = std::max<uint64_t>(need, 1u << 20);
+    HIPCHK(alloc_queue(c->qa, cap));
+    HIPCHK(alloc_queue(c->qb, cap));
+    c->qcap = cap;
+    return PCRHIP_OK;
+}
+
+int validate_job(const pcrhip_scene *s, const pcrhip_job_desc *d) {
+    if (!s || !d) return fail(PCRHIP_EINVAL, "null scene or job descriptor");
+    if (d->width < 1 || d->height < 1 || d->depth < 0 || d->spp < 1 || d->shadow < 1)
+        return fail(PCRHIP_EINVAL, "width, height, spp and shadow must be >= 1 and depth >= 0");
+    if (d->depth > kMaxDepthBounces) return fail(PCRHIP_EINVAL, "depth exceeds 255");
+    if ((d->flags & PCRHIP_SPECTRAL) && (d->hero < 1 || d->hero > 4))
+        return fail(PCRHIP_EINVAL, "hero must be in 1..4");
+    int cw = d->cw ? d->cw : d->width, ch = d->ch ? d->ch : d->height;
+    if (d->x0 < 0 || d->y0 < 0 || cw < 1 || ch < 1 || d->x0 + cw > d->width || d->y0 + ch > d->height)
+        return fail(PCRHIP_EINVAL, "crop window outside the frame");
+    if ((uint64_t)d->width * d->height * (uint64_t)d->spp >= (1ull << 62))
+        return fail(PCRHIP_EINVAL, "too many samples");
+    if (s->S.n_lights < 1) return fail(PCRHIP_EINVAL, "scene has no light");
+    return PCRHIP_OK;
+}
+
+void fill_djob(const pcrhip_job *j, DJob &J) {
+    const pcrhip_job_desc &d = j->d;
+    J.S = j->scene->S;
+    J.W = d.width;
+    J.H = d.height;
+    J.x0 = d.x0;
+    J.y0 = d.y0;
+    J.cw = j->cw;
+    J.ch = j->ch;
+    J.depth = d.depth;
+    J.shadow = d.shadow;
+    J.spp = d.spp;
+    J.hero = (d.flags & PCRHIP_SPECTRAL) ? d.hero : 3;
+    J.flags = d.flags;
+    J.strata = std::max(1, (int)std::round(std::sqrt((float)d.spp)));
+    J.seed = d.seed;
+    // Renderer.cpp:149-150 (host libm tanf, as the reference)
+    J.aspect = d.width / (float)d.height;
+    J.scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * j->scene->S.fov);
+    J.spp_pass = j->spp_pass;
+    J.s_begin = j->s_next;
+    J.radiance = j->radiance;
+    J.accum = j->accum;
+}
+
+void finish_job(pcrhip_ctx *c, pcrhip_job *j, int status, const std::string &err) {
+    // caller holds c->m
+    j->status = status;
+    j->error = err;
+    j->state = pcrhip_job::DONE;
+    j->t_done = Clock::now();
+    if (j->slot >= 0) c->active[j->slot] = nullptr;
+    j->slot = -1;
+    (void)hipFree(j->accum);
+    (void)hipFree(j->radiance);
+    j->accum = nullptr;
+    j->radiance = nullptr;
+    c->mem_in_use -= j->bytes;
+    j->bytes = 0;
+    j->scene->jobs--;
+    c->done_cv.notify_all();
+}
+
+// Admits queued jobs into free slots while the memory budget allows.
+void admit(pcrhip_ctx *c) {
+    // highest priority first, FIFO within a priority
+    std::stable_sort(c->pending.begin(), c->pending.end(),
+                     [](const pcrhip_job *a, const pcrhip_job *b) { return a->d.priority > b->d.priority; });
+    int active = 0;
+    for (auto *j : c->active) active += j != nullptr;
+    while (!c->pending.empty() && active < std::min(c->cfg.max_active_jobs, kSlots)) {
+        pcrhip_job *j = c->pending.front();
+        if (j->cancel.load()) {
+            c->pending.pop_front();
+            finish_job(c, j, PCRHIP_ECANCELLED, "cancelled before admission");
+            continue;
+        }
+        uint64_t pix = (uint64_t)j->cw * j->ch;
+        uint64_t spp_pass = std::max<uint64_t>(1, std::min<uint64_t>(j->d.spp, c->cfg.target_pass_paths / pix));
+        uint64_t bytes = pix * 3 * sizeof(double) + pix * spp_pass * sizeof(float4);
+        if (active > 0 && c->mem_in_use + bytes > c->cfg.memory_budget) break;  // wait for memory
+        if (bytes > c->cfg.memory_budget) {
+            c->pending.pop_front();
+            finish_job(c, j, PCRHIP_ENOMEM, "job needs more memory than the admission budget");
+            continue;
+        }
+        int slot = -1;
+        for (int s = 0; s < kSlots; s++)
+            if (!c->active[s]) { slot = s; break; }
+        if (slot < 0) break;
+        hipError_t e1 = hipMalloc(&j->accum, pix * 3 * sizeof(double));
+        hipError_t e2 = e1 == hipSuccess ? hipMalloc(&j->radiance, pix * spp_pass * sizeof(float4)) : e1;
+        hipError_t e3 = e2 == hipSuccess ? hipMemsetAsync(j->accum, 0, pix * 3 * sizeof(double), c->stream) : e2;
+        c->pending.pop_front();
+        j->bytes = bytes;
+        c->mem_in_use += bytes;
+        if (e3 != hipSuccess) {
+            finish_job(c, j, PCRHIP_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e3));
+            continue;
+        }
+        j->rad_paths = pix * spp_pass;
+        j->slot = slot;
+        j->state = pcrhip_job::RUNNING;
+        j->t_admit = Clock::now();
+        c->active[slot] = j;
+        active++;
+    }
+}
+
+// Runs one pass over every active job.  Returns a status; on a device error the
+// context is unusable and every job fails.
+int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs) {
+    std::vector<GenSeg> segs;
+    std::vector<ResolveSeg> rsegs;
+    uint64_t n_gen = 0, n_pix = 0;
+    int max_depth = 0, stack = 8;
+    std::vector<int32_t> slots;
+    for (auto *j : jobs) {
+        uint64_t pix = (uint64_t)j->cw * j->ch;
+        int spp_pass = (int)std::max<uint64_t>(1, std::min<uint64_t>(j->d.spp - j->s_next, j->rad_paths / pix));
+        j->spp_pass = spp_pass;
+        fill_djob(j, c->jobs_host[j->slot]);
+        segs.push_back(GenSeg{j->slot, (uint32_t)n_gen, (uint32_t)(pix * spp_pass), 0u});
+        rsegs.push_back(ResolveSeg{j->slot, (uint32_t)n_pix, (uint32_t)pix, 0u});
+        n_gen += pix * spp_pass;
+        n_pix += pix;
+        max_depth = std::max(max_depth, j->d.depth);
+        stack = std::max(stack, j->scene->stack);
+        slots.push_back(j->slot);
+    }
+    if (n_gen >= (1ull << 31)) return fail(PCRHIP_EINVAL, "pass too large");
+    int rc = ensure_queues(c, 4 * n_gen);  // a dispersion fork emits up to 4 rays per path
+    if (rc) return rc;
+    for (auto *j : jobs)
+        HIPCHK(hipMemcpyAsync(c->jobs_dev + j->slot, c->jobs_host + j->slot, sizeof(DJob), hipMemcpyHostToDevice,
+                              c->stream));
+    HIPCHK(hipMemcpyAsync(c->segs_dev, segs.data(), segs.size() * sizeof(GenSeg), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->rsegs_dev, rsegs.data(), rsegs.size() * sizeof(ResolveSeg), hipMemcpyHostToDevice,
+                          c->stream));
+    int nb = std::max(max_depth, 1);
+    HIPCHK(pcr_launch_reset(c->ctrs, 2 * nb + 2, c->stream));
+    HIPCHK(hipMemsetAsync(c->slot_counters, 0, sizeof(unsigned long long) * kSlots * 4, c->stream));
+    const bool prof = c->profile.load();
+    std::vector<hipEvent_t> ev;
+    if (prof) {
+        ev.resize(2 * nb + 2);
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    const int grid = c->n_cu * std::max(1, c->cfg.blocks_per_cu);
+    for (int b = 0; b < nb; b++) {
+        BounceArgs a{};
+        a.jobs = c->jobs_dev;
+        a.segs = c->segs_dev;
+        a.n_segs = (int)segs.size();
+        a.n_gen = (uint32_t)n_gen;
+        a.in = (b & 1) ? c->qa : c->qb;
+        a.out = (b & 1) ? c->qb : c->qa;
+        a.count_in = b == 0 ? nullptr : c->ctrs + 2 * (b - 1);
+        a.count_out = c->ctrs + 2 * b;
+        a.work = c->ctrs + 2 * b + 1;
+        a.capacity = (uint32_t)std::min<uint64_t>(c->qcap, 0xFFFFFFFFu);
+        a.overflow = c->ctrs + 2 * nb;
+        a.counters = c->slot_counters;
+        if (prof) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
+        HIPCHK(pcr_launch_bounce(&a, b == 0, stack, grid, c->cfg.block, c->stream));
+        if (prof) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
+    }
+    ResolveArgs ra{c->jobs_dev, c->rsegs_dev, (int)rsegs.size()};
+    if (prof) HIPCHK(hipEventRecord(ev[2 * nb], c->stream));
+    HIPCHK(pcr_launch_resolve(&ra, (uint32_t)n_pix, c->stream));
+    if (prof) HIPCHK(hipEventRecord(ev[2 * nb + 1], c->stream));
+    std::vector<uint32_t> ctr(2 * nb + 2);
+    HIPCHK(hipMemcpyAsync(ctr.data(), c->ctrs, ctr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> sc(kSlots * 4);
+    HIPCHK(hipMemcpyAsync(sc.data(), c->slot_counters, sc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ctr[2 * nb]) return fail(PCRHIP_EDEVICE, "ray queue overflow");
+    if (prof) {
+        std::lock_guard<std::mutex> lk(c->pm);
+        uint64_t rays_in = n_gen;  // generating bounce reads no queue
+        for (int b = 0; b < nb; b++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]));
+            c->prof.bounce_ms += ms;
+            c->prof.bounce_launches++;
+            uint64_t written = ctr[2 * b];
+            c->prof.queue_bytes += (b == 0 ? 0 : rays_in * 80) + written * 80;
+            rays_in = written;
+        }
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[2 * nb], ev[2 * nb + 1]));
+        c->prof.resolve_ms += ms;
+        c->prof.resolve_launches++;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    for (auto *j : jobs) {
+        for (int k = 0; k < 4; k++) j->counters[k] += sc[j->slot * 4 + k];
+        j->s_next += j->spp_pass;
+        j->passes++;
+    }
+    return PCRHIP_OK;
+}
+
+void worker_main(pcrhip_ctx *c) {
+    (void)hipSetDevice(c->device);
+    std::unique_lock<std::mutex> lk(c->m);
+    for (;;) {
+        c->cv.wait(lk, [&] {
+            if (c->stop) return true;
+            if (!c->pending.empty()) return true;
+            for (auto *j : c->active)
+                if (j) return true;
+            return false;
+        });
+        if (c->stop) {
+            // the context is going away: every queued or running job ends cancelled
+            for (auto *j : c->pending) finish_job(c, j, PCRHIP_ECANCELLED, "context destroyed");
+            c->pending.clear();
+            for (auto *j : c->active)
+                if (j) finish_job(c, j, PCRHIP_ECANCELLED, "context destroyed");
+            break;
+        }
+        admit(c);
+        std::vector<pcrhip_job *> run;
+        for (auto *j : c->active)
+            if (j) {
+                if (j->cancel.load()) finish_job(c, j, PCRHIP_ECANCELLED, "cancelled");
+                else run.push_back(j);
+            }
+        if (run.empty()) continue;
+        lk.unlock();
+        int rc = run_pass(c, run);
+        std::string err = rc ? g_err : std::string();
+        lk.lock();
+        if (rc) {
+            for (auto *j : run)
+                if (j->state == pcrhip_job::RUNNING) finish_job(c, j, rc, err);
+            continue;
+        }
+        for (auto *j : run) {
+            if (j->state == pcrhip_job::RUNNING && j->s_next >= j->d.spp) {
+                uint64_t n = (uint64_t)j->cw * j->ch * 3;
+                j->sums.resize(n);
+                hipError_t e = hipMemcpy(j->sums.data(), j->accum, n * sizeof(double), hipMemcpyDeviceToHost);
+                finish_job(c, j, e == hipSuccess ? PCRHIP_OK : PCRHIP_EDEVICE,
+                           e == hipSuccess ? "" : hipGetErrorString(e));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- scene upload --
+template <class T>
+int upload(pcrhip_scene *s, const std::vector<T> &v, const T **dst) {
+    if (v.empty()) {
+        *dst = nullptr;
+        return PCRHIP_OK;
+    }
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, v.size() * sizeof(T)));
+    s->allocs.push_back(p);
+    s->bytes += v.size() * sizeof(T);
+    HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (const T *)p;
+    return PCRHIP_OK;
+}
+
+int build_scene(pcrhip_scene *s, const pcrhip_scene_desc *d) {
+    DScene &S = s->S;
+    if (d->n_materials < 1) return fail(PCRHIP_EINVAL, "scene needs at least one material");
+    auto mat_ok = [&](int m) { return m >= 0 && m < d->n_materials; };
+    // materials
+    std::vector<DMat> mats(d->n_materials);
+    std::vector<float> tabs;
+    for (int i = 0; i < d->n_materials; i++) {
+        DMat &m = mats[i];
+        std::memset(&m, 0, sizeof(m));
+        const float *a = d->mat_albedo + 3 * i, *e = d->mat_emissive + 3 * i;
+        m.albedo = make_float4(a[0], a[1], a[2], d->mat_ior[i]);
+        m.emissive = make_float4(e[0], e[1], e[2], d->mat_cauchy_b[i]);
+        const float *fa = d->mat_fit_albedo + 4 * i, *fe = d->mat_fit_emissive + 4 * i;
+        m.fitA = make_float4(fa[0], fa[1], fa[2], fa[3]);
+        m.fitE = make_float4(fe[0], fe[1], fe[2], fe[3]);
+        uint32_t fl = d->mat_flags[i];
+        m.flags = 0;
+        if (fl & PCRHIP_MAT_METALLIC) m.flags |= MF_METALLIC;
+        if (fl & PCRHIP_MAT_TRANSPARENT) m.flags |= MF_TRANSPARENT;
+        if ((fl & PCRHIP_MAT_TAB_ALBEDO) && d->mat_tab_albedo) m.flags |= MF_TAB_ALBEDO;
+        if ((fl & PCRHIP_MAT_TAB_EMISSIVE) && d->mat_tab_emissive) m.flags |= MF_TAB_EMISSIVE;
+        if (e[0] > 0 || e[1] > 0 || e[2] > 0) m.flags |= MF_EMISSIVE;  // Material::isEmissive
+        m.tab = -1;
+        if (m.flags & (MF_TAB_ALBEDO | MF_TAB_EMISSIVE)) {
+            m.tab = (int32_t)tabs.size();
+            for (int k = 0; k < 61; k++) tabs.push_back((m.flags & MF_TAB_ALBEDO) ? d->mat_tab_albedo[61 * i + k] : 0.f);
+            for (int k = 0; k < 61; k++)
+                tabs.push_back((m.flags & MF_TAB_EMISSIVE) ? d->mat_tab_emissive[61 * i + k] : 0.f);
+        }
+    }
+    // spheres
+    std::vector<float4> sph(d->n_spheres);
+    std::vector<int32_t> sphm(d->n_spheres);
+    for (int i = 0; i < d->n_spheres; i++) {
+        const float *p = d->spheres + 4 * i;
+        sph[i] = make_float4(p[0], p[1], p[2], p[3]);
+        sphm[i] = d->sphere_mat[i];
+        if (!mat_ok(sphm[i])) return fail(PCRHIP_EINVAL, "sphere material out of range");
+    }
+    // planes (Plane.h:19-24: N = normalize(u x v))
+    std::vector<float4> pl(4 * (size_t)d->n_planes);
+    for (int i = 0; i < d->n_planes; i++) {
+        const float *p = d->planes + 9 * i;
+        H3 o = h3(p), u = h3(p + 3), v = h3(p + 6);
+        H3 N = hnorm(hcross(u, v));
+        if (!mat_ok(d->plane_mat[i])) return fail(PCRHIP_EINVAL, "plane material out of range");
+        pl[4 * i] = f4(o, hdot(u, u));
+        pl[4 * i + 1] = f4(u, hdot(v, v));
+        pl[4 * i + 2] = f4(v, ibits(d->plane_mat[i]));
+        pl[4 * i + 3] = f4(N, 0.f);
+    }
+    // triangles (Triangle.h:23-25: flatN = normalize(e1 x e2))
+    std::vector<float4> tri(3 * (size_t)d->n_triangles), trin(4 * (size_t)d->n_triangles);
+    for (int i = 0; i < d->n_triangles; i++) {
+        const float *p = d->triangles + 9 * i;
+        H3 v0 = h3(p), v1 = h3(p + 3), v2 = h3(p + 6);
+        H3 e1 = hsub(v1, v0), e2 = hsub(v2, v0);
+        bool smooth = d->tri_smooth && d->tri_smooth[i] && d->tri_normals;
+        if (!mat_ok(d->tri_mat[i])) return fail(PCRHIP_EINVAL, "triangle material out of range");
+        tri[3 * i] = f4(v0, ibits(d->tri_mat[i]));
+        tri[3 * i + 1] = f4(e1, ibits(smooth ? 1 : 0));
+        tri[3 * i + 2] = f4(e2, 0.f);
+        if (smooth) {
+            const float *n = d->tri_normals + 9 * i;
+            trin[4 * i] = f4(h3(n), 0.f);
+            trin[4 * i + 1] = f4(h3(n + 3), 0.f);
+            trin[4 * i + 2] = f4(h3(n + 6), 0.f);
+        } else {
+            trin[4 * i] = trin[4 * i + 1] = trin[4 * i + 2] = make_float4(0, 0, 0, 0);
+        }
+        trin[4 * i + 3] = f4(hnorm(hcross(e1, e2)), 0.f);
+    }
+    // BVH: reference preorder nodes -> inner nodes holding both child boxes
+    std::vector<float4> bvh;
+    std::vector<int2> leaves;
+    S.has_bvh = 0;
+    S.root_ref = 0;
+    int depth = 0;
+    if (d->n_nodes > 0) {
+        if (!d->nodes) return fail(PCRHIP_EINVAL, "nodes missing");
+        const int32_t *N = d->nodes;
+        auto box = [&](int n, int k) { return ibits(N[9 * n + k]); };
+        std::vector<int32_t> ref(d->n_nodes, 0);
+        // assign refs: inner nodes numbered in preorder, leaves in preorder
+        int ni = 0, nl = 0;
+        for (int n = 0; n < d->n_nodes; n++) {
+            int cnt = N[9 * n + 8];
+            if (cnt > 0) {
+                int first = N[9 * n + 6];
+                if (first < 0 || first + cnt > d->n_triangles) return fail(PCRHIP_EINVAL, "leaf range out of bounds");
+                ref[n] = ~nl++;
+                leaves.push_back(make_int2(first, cnt));
+            } else {
+                ref[n] = ni++;
+            }
+        }
+        bvh.resize(4 * (size_t)ni);
+        for (int n = 0; n < d->n_nodes; n++) {
+            if (N[9 * n + 8] > 0) continue;
+            int l = N[9 * n + 6], r = N[9 * n + 7];
+            if (l <= 0 || r <= 0 || l >= d->n_nodes || r >= d->n_nodes) return fail(PCRHIP_EINVAL, "bad child index");
+            int i = ref[n];
+            bvh[4 * i] = make_float4(box(l, 0), box(l, 1), box(l, 2), box(l, 3));
+            bvh[4 * i + 1] = make_float4(box(l, 4), box(l, 5), box(r, 0), box(r, 1));
+            bvh[4 * i + 2] = make_float4(box(r, 2), box(r, 3), box(r, 4), box(r, 5));
+            bvh[4 * i + 3] = make_float4(ibits(ref[l]), ibits(ref[r]), 0.f, 0.f);
+        }
+        // depth (longest root-to-leaf chain, Bvh.cpp:340-349), iteratively
+        std::vector<std::pair<int, int>> stk{{0, 1}};
+        while (!stk.empty()) {
+            auto [n, dd] = stk.back();
+            stk.pop_back();
+            depth = std::max(depth, dd);
+            if (dd > 4096) return fail(PCRHIP_EINVAL, "BVH too deep or cyclic");
+            if (N[9 * n + 8] == 0) {
+                stk.push_back({N[9 * n + 6], dd + 1});
+                stk.push_back({N[9 * n + 7], dd + 1});
+            }
+        }
+        S.has_bvh = 1;
+        S.root_ref = ref[0];
+        S.root_lo = make_float4(box(0, 0), box(0, 1), box(0, 2), 0.f);
+        S.root_hi = make_float4(box(0, 3), box(0, 4), box(0, 5), 0.f);
+        s->n_inner = ni;
+        s->n_leaves = nl;
+    }
+    s->depth = depth;
+    // traversal stack: at most depth-1 pending far children below the current node
+    s->stack = depth <= 8 ? 8 : depth <= 16 ? 16 : depth <= 32 ? 32 : 64;
+    // lights
+    std::vector<DLight> lights(d->n_lights);
+    std::vector<float4> ltri(4 * (size_t)d->n_light_tris);
+    for (int i = 0; i < d->n_light_tris; i++) {
+        const float *p = d->light_tris + 9 * i;
+        H3 v0 = h3(p), v1 = h3(p + 3), v2 = h3(p + 6);
+        H3 e1 = hsub(v1, v0), e2 = hsub(v2, v0);
+        if (!mat_ok(d->light_tri_mat[i])) return fail(PCRHIP_EINVAL, "light triangle material out of range");
+        ltri[4 * i] = f4(v0, ibits(d->light_tri_mat[i]));
+        ltri[4 * i + 1] = f4(e1, d->light_tri_cum[i]);
+        ltri[4 * i + 2] = f4(e2, 0.f);
+        ltri[4 * i + 3] = f4(hnorm(hcross(e1, e2)), 0.f);
+    }
+    float total = 0.f;
+    for (int i = 0; i < d->n_lights; i++) {
+        DLight &L = lights[i];
+        std::memset(&L, 0, sizeof(L));
+        L.kind = d->light_kind[i];
+        L.first = d->light_first[i];
+        L.count = d->light_count[i];
+        L.area = d->light_area[i];
+        total += L.area;  // Renderer.cpp:134-136
+        if (L.kind == 0) {
+            if (L.first < 0 || L.first >= d->n_light_planes) return fail(PCRHIP_EINVAL, "light plane out of range");
+            const float *p = d->light_planes + 9 * L.first;
+            H3 o = h3(p), u = h3(p + 3), v = h3(p + 6);
+            if (!mat_ok(d->light_plane_mat[L.first])) return fail(PCRHIP_EINVAL, "light material out of range");
+            L.o = f4(o, ibits(d->light_plane_mat[L.first]));
+            L.u = f4(u, 0.f);
+            L.v = f4(v, 0.f);
+            L.n = f4(hnorm(hcross(u, v)), 0.f);
+        } else if (L.kind == 1) {
+            if (L.first < 0 || L.count < 1 || L.first + L.count > d->n_light_tris)
+                return fail(PCRHIP_EINVAL, "light triangle range out of bounds");
+        } else {
+            return fail(PCRHIP_EINVAL, "unknown light kind");
+        }
+    }
+    int rc;
+    if ((rc = upload(s, mats, &S.mat))) return rc;
+    if ((rc = upload(s, tabs, &S.tabs))) return rc;
+    if ((rc = upload(s, sph, &S.sph))) return rc;
+    if ((rc = upload(s, sphm, &S.sph_mat))) return rc;
+    if ((rc = upload(s, pl, &S.pl))) return rc;
+    if ((rc = upload(s, tri, &S.tri))) return rc;
+    if ((rc = upload(s, trin, &S.tri_n))) return rc;
+    if ((rc = upload(s, bvh, &S.bvh))) return rc;
+    if ((rc = upload(s, leaves, &S.leaves))) return rc;
+    if ((rc = upload(s, lights, &S.lights))) return rc;
+    if ((rc = upload(s, ltri, &S.ltri))) return rc;
+    if (d->tri_orig && d->n_triangles) {
+        std::vector<int32_t> orig(d->tri_orig, d->tri_orig + d->n_triangles);
+        const int32_t *p = nullptr;
+        if ((rc = upload(s, orig, &p))) return rc;
+        s->tri_orig = (int32_t *)p;
+    }
+    S.n_sph = d->n_spheres;
+    S.n_pl = d->n_planes;
+    S.n_tri = d->n_triangles;
+    S.n_inner = s->n_inner;
+    S.n_leaves = s->n_leaves;
+    S.n_mat = d->n_materials;
+    S.n_lights = d->n_lights;
+    S.total_area = total;
+    S.cam = make_float3(d->cam_pos[0], d->cam_pos[1], d->cam_pos[2]);
+    S.fov = d->cam_fov;
+    return PCRHIP_OK;
+}
+
+void free_scene(pcrhip_scene *s) {
+    for (void *p : s->allocs) (void)hipFree(p);
+    s->allocs.clear();
+}
+
+}  // namespace
+
+// ================================================================== C ABI ==
+extern "C" {
+
+const char *pcrhip_last_error(void) { return g_err.c_str(); }
+int pcrhip_abi_version(void) { return PCRHIP_ABI_VERSION; }
+
+int pcrhip_create(int device, pcrhip_ctx **out) {
+    if (!out) return fail(PCRHIP_EINVAL, "out is null");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(PCRHIP_EINVAL, "no such device");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    auto *c = new pcrhip_ctx();
+    c->device = device;
+    c->n_cu = prop.multiProcessorCount;
+    c->cfg.block = 128;
+    c->cfg.blocks_per_cu = 8;
+    c->cfg.target_pass_paths = 4u << 20;
+    c->cfg.memory_budget = 64ull << 30;
+    c->cfg.max_active_jobs = 32;
+    auto cleanup = [&](int rc) {
+        (void)hipStreamDestroy(c->stream);
+        (void)hipFree(c->jobs_dev);
+        (void)hipFree(c->segs_dev);
+        (void)hipFree(c->rsegs_dev);
+        (void)hipFree(c->ctrs);
+        (void)hipFree(c->slot_counters);
+        (void)hipHostFree(c->jobs_host);
+        delete c;
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&c->jobs_dev, kSlots * sizeof(DJob))) != hipSuccess ||
+        (e = hipMalloc(&c->segs_dev, kMaxSegs * sizeof(GenSeg))) != hipSuccess ||
+        (e = hipMalloc(&c->rsegs_dev, kMaxSegs * sizeof(ResolveSeg))) != hipSuccess ||
+        (e = hipMalloc(&c->ctrs, (2 * kMaxDepthBounces + 2) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&c->slot_counters, kSlots * 4 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipHostMalloc(&c->jobs_host, kSlots * sizeof(DJob), 0)) != hipSuccess)
+        return cleanup(fail(PCRHIP_EDEVICE, std::string("context allocation failed: ") + hipGetErrorString(e)));
+    std::memset(c->jobs_host, 0, kSlots * sizeof(DJob));
+    c->worker = std::thread(worker_main, c);
+    *out = c;
+    return PCRHIP_OK;
+}
+
+int pcrhip_destroy(pcrhip_ctx *c) {
+    if (!c) return fail(PCRHIP_EINVAL, "null context");
+    {
+        std::lock_guard<std::mutex> lk(c->m);
+        c->stop = true;
+    }
+    c->cv.notify_all();
+    if (c->worker.joinable()) c->worker.join();
+    (void)hipSetDevice(c->device);
+    free_queue(c->qa);
+    free_queue(c->qb);
+    (void)hipFree(c->jobs_dev);
+    (void)hipFree(c->segs_dev);
+    (void)hipFree(c->rsegs_dev);
+    (void)hipFree(c->ctrs);
+    (void)hipFree(c->slot_counters);
+    (void)hipHostFree(c->jobs_host);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PCRHIP_OK;
+}
+
+int pcrhip_get_config(pcrhip_ctx *c, pcrhip_config *out) {
+    if (!c || !out) return fail(PCRHIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(c->m);
+    *out = c->cfg;
+    return PCRHIP_OK;
+}
+int pcrhip_set_config(pcrhip_ctx *c, const pcrhip_config *cfg) {
+    if (!c || !cfg) return fail(PCRHIP_EINVAL, "null argument");
+    if (cfg->block != 64 && cfg->block != 128 && cfg->block != 256) return fail(PCRHIP_EINVAL, "block must be 64/128/256");
+    if (cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64) return fail(PCRHIP_EINVAL, "blocks_per_cu out of range");
+    if (cfg->target_pass_paths < 1024) return fail(PCRHIP_EINVAL, "target_pass_paths too small");
+    if (cfg->max_active_jobs < 1 || cfg->max_active_jobs > kSlots) return fail(PCRHIP_EINVAL, "max_active_jobs out of range");
+    std::lock_guard<std::mutex> lk(c->m);
+    c->cfg = *cfg;
+    return PCRHIP_OK;
+}
+
+int pcrhip_scene_upload(pcrhip_ctx *c, const pcrhip_scene_desc *d, pcrhip_scene **out) {
+    if (!c || !d || !out) return fail(PCRHIP_EINVAL, "null argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    auto *s = new pcrhip_scene();
+    s->ctx = c;
+    int rc = build_scene(s, d);
+    if (rc) {
+        free_scene(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PCRHIP_OK;
+}
+
+int pcrhip_scene_release(pcrhip_scene *s) {
+    if (!s) return fail(PCRHIP_EINVAL, "null scene");
+    if (s->jobs.load() > 0) return fail(PCRHIP_ESTATE, "scene still used by queued or running jobs");
+    (void)hipSetDevice(s->ctx->device);
+    free_scene(s);
+    delete s;
+    return PCRHIP_OK;
+}
+
+int pcrhip_scene_get_info(pcrhip_scene *s, pcrhip_scene_info *out) {
+    if (!s || !out) return fail(PCRHIP_EINVAL, "null argument");
+    out->n_inner_nodes = s->n_inner;
+    out->n_leaves = s->n_leaves;
+    out->bvh_depth = s->depth;
+    out->traversal_stack = s->stack;
+    out->device_bytes = s->bytes;
+    return PCRHIP_OK;
+}
+
+int pcrhip_intersect(pcrhip_scene *s, const float *rays, int n, float tmax, float *t, int32_t *kind, int32_t *prim,
+                     int32_t *mat, float *hit, float *normal) {
+    if (!s || (n > 0 && (!rays || !t || !kind || !prim || !mat))) return fail(PCRHIP_EINVAL, "null argument");
+    if (n < 0) return fail(PCRHIP_EINVAL, "negative ray count");
+    if (n == 0) return PCRHIP_OK;
+    pcrhip_ctx *c = s->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    size_t N = (size_t)n;
+    char *buf = nullptr;
+    size_t bytes = N * (6 * 4 + 4 + 4 + 4 + 4 + 12 + 12);
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(st);
+        return fail(PCRHIP_ENOMEM, hipGetErrorString(e));
+    }
+    IntersectArgs a{};
+    a.S = s->S;
+    a.rays = (const float *)buf;
+    a.n = (uint32_t)n;
+    a.tmax = tmax;
+    a.tri_orig = s->tri_orig;
+    a.t = (float *)(buf + N * 24);
+    a.kind = (int32_t *)(buf + N * 28);
+    a.prim = (int32_t *)(buf + N * 32);
+    a.mat = (int32_t *)(buf + N * 36);
+    a.hit = (float *)(buf + N * 40);
+    a.normal = (float *)(buf + N * 52);
+    int rc = PCRHIP_OK;
+    auto step = [&](hipError_t err, const char *what) {
+        if (rc == PCRHIP_OK && err != hipSuccess) rc = fail(PCRHIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(err));
+    };
+    step(hipMemcpyAsync(buf, rays, N * 24, hipMemcpyHostToDevice, st), "upload rays");
+    if (rc == PCRHIP_OK) step(pcr_launch_intersect(&a, s->stack, st), "intersect kernel");
+    if (rc == PCRHIP_OK) {
+        step(hipMemcpyAsync(t, a.t, N * 4, hipMemcpyDeviceToHost, st), "t");
+        step(hipMemcpyAsync(kind, a.kind, N * 4, hipMemcpyDeviceToHost, st), "kind");
+        step(hipMemcpyAsync(prim, a.prim, N * 4, hipMemcpyDeviceToHost, st), "prim");
+        step(hipMemcpyAsync(mat, a.mat, N * 4, hipMemcpyDeviceToHost, st), "mat");
+        if (hit) step(hipMemcpyAsync(hit, a.hit, N * 12, hipMemcpyDeviceToHost, st), "hit");
+        if (normal) step(hipMemcpyAsync(normal, a.normal, N * 12, hipMemcpyDeviceToHost, st), "normal");
+        step(hipStreamSynchronize(st), "sync");
+    }
+    (void)hipFree(buf);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int pcrhip_job_submit(pcrhip_ctx *c, pcrhip_scene *s, const pcrhip_job_desc *d, pcrhip_job **out) {
+    if (!c || !out) return fail(PCRHIP_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = validate_job(s, d);
+    if (rc) return rc;
+    if (s->ctx != c) return fail(PCRHIP_EINVAL, "scene belongs to another context");
+    auto *j = new pcrhip_job();
+    j->ctx = c;
+    j->scene = s;
+    j->d = *d;
+    j->cw = d->cw ? d->cw : d->width;
+    j->ch = d->ch ? d->ch : d->height;
+    if (!d->cw || !d->ch) {
+        j->d.x0 = j->d.y0 = 0;
+        j->cw = d->width;
+        j->ch = d->height;
+    }
+    j->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(c->m);
+        if (!c->fatal.empty()) {
+            delete j;
+            return fail(PCRHIP_EDEVICE, c->fatal);
+        }
+        s->jobs++;  // released by finish_job
+        c->pending.push_back(j);
+    }
+    c->cv.notify_all();
+    *out = j;
+    return PCRHIP_OK;
+}
+
+int pcrhip_job_progress(pcrhip_job *j, int *done, int *total) {
+    if (!j) return fail(PCRHIP_EINVAL, "null job");
+    std::lock_guard<std::mutex> lk(j->ctx->m);
+    int rows = j->ch;
+    int dn = j->state == pcrhip_job::DONE && j->status == PCRHIP_OK
+                 ? rows
+                 : (int)((int64_t)rows * j->s_next / std::max(1, j->d.spp));
+    if (done) *done = dn;
+    if (total) *total = rows;
+    return PCRHIP_OK;
+}
+
+int pcrhip_job_cancel(pcrhip_job *j) {
+    if (!j) return fail(PCRHIP_EINVAL, "null job");
+    j->cancel.store(true);
+    j->ctx->cv.notify_all();
+    return PCRHIP_OK;
+}
+
+int pcrhip_job_wait(pcrhip_job *j, int timeout_ms) {
+    if (!j) return fail(PCRHIP_EINVAL, "null job");
+    pcrhip_ctx *c = j->ctx;
+    std::unique_lock<std::mutex> lk(c->m);
+    auto pred = [&] { return j->state == pcrhip_job::DONE; };
+    if (timeout_ms < 0) c->done_cv.wait(lk, pred);
+    else if (!c->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred))
+        return fail(PCRHIP_ETIMEOUT, "job still running");
+    if (j->status != PCRHIP_OK) return fail(j->status, j->error);
+    return PCRHIP_OK;
+}
+
+static int job_stats(pcrhip_job *j, pcrhip_stats *st) {
+    if (!st) return PCRHIP_OK;
+    st->paths = j->counters[0];
+    st->ext_rays = j->counters[1];
+    st->shadow_rays = j->counters[2];
+    st->emit_hits = j->counters[3];
+    st->ms_queue = ms_between(j->t_submit, j->t_admit);
+    st->ms_render = ms_between(j->t_admit, j->t_done);
+    st->ms_total = ms_between(j->t_submit, j->t_done);
+    st->passes = j->passes;
+    return PCRHIP_OK;
+}
+
+int pcrhip_job_result(pcrhip_job *j, float *out, size_t n, pcrhip_stats *st) {
+    if (!j) return fail(PCRHIP_EINVAL, "null job");
+    std::lock_guard<std::mutex> lk(j->ctx->m);
+    if (j->state != pcrhip_job::DONE) return fail(PCRHIP_ESTATE, "job not finished");
+    if (j->status != PCRHIP_OK) return fail(j->status, j->error);
+    size_t need = (size_t)j->cw * j->ch * 3;
+    if (out) {
+        if (n < need) return fail(PCRHIP_EINVAL, "output buffer too small");
+        for (size_t i = 0; i < need; i++) out[i] = (float)(j->sums[i] / (double)j->d.spp);
+    }
+    return job_stats(j, st);
+}
+
+int pcrhip_job_result_sums(pcrhip_job *j, double *out, size_t n) {
+    if (!j || !out) return fail(PCRHIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(j->ctx->m);
+    if (j->state != pcrhip_job::DONE) return fail(PCRHIP_ESTATE, "job not finished");
+    if (j->status != PCRHIP_OK) return fail(j->status, j->error);
+    size_t need = (size_t)j->cw * j->ch * 3;
+    if (n < need) return fail(PCRHIP_EINVAL, "output buffer too small");
+    std::memcpy(out, j->sums.data(), need * sizeof(double));
+    return PCRHIP_OK;
+}
+
+int pcrhip_job_release(pcrhip_job *j) {
+    if (!j) return fail(PCRHIP_EINVAL, "null job");
+    pcrhip_ctx *c = j->ctx;
+    {
+        std::unique_lock<std::mutex> lk(c->m);
+        if (j->state != pcrhip_job::DONE) {
+            // release of an unfinished job cancels it and waits for the pass boundary
+            j->cancel.store(true);
+            c->cv.notify_all();
+            c->done_cv.wait(lk, [&] { return j->state == pcrhip_job::DONE; });
+        }
+    }
+    delete j;
+    return PCRHIP_OK;
+}
+
+int pcrhip_render(pcrhip_ctx *c, pcrhip_scene *s, const pcrhip_job_desc *d, float *out, size_t n, pcrhip_stats *st) {
+    pcrhip_job *j = nullptr;
+    int rc = pcrhip_job_submit(c, s, d, &j);
+    if (rc) return rc;
+    rc = pcrhip_job_wait(j, -1);
+    if (rc == PCRHIP_OK) rc = pcrhip_job_result(j, out, n, st);
+    std::string err = rc ? g_err : std::string();
+    pcrhip_job_release(j);
+    if (rc) g_err = err;
+    return rc;
+}
+
+int pcrhip_profile_enable(pcrhip_ctx *c, int enable) {
+    if (!c) return fail(PCRHIP_EINVAL, "null context");
+    c->profile.store(enable != 0);
+    return PCRHIP_OK;
+}
+int pcrhip_profile_get(pcrhip_ctx *c, pcrhip_profile *out, int reset) {
+    if (!c || !out) return fail(PCRHIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(c->pm);
+    *out = c->prof;
+    if (reset) c->prof = pcrhip_profile{};
+    return PCRHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,683 @@
+// wavefront.hip -- gfx950 wavefront path tracer kernels.
+//
+// Organisation (one pass = one batch of paths, depth bounces):
+//   k_bounce<FIRST=true>   generates camera rays for the pass (NumGen stream per
+//                          path) and runs bounce 0 in the same kernel;
+//   k_bounce<FIRST=false>  bounce b > 0 over the compacted SoA queue of live rays;
+//   k_resolve              per pixel: radiance -> CIE XYZ (or RGB), summed into
+//                          the FP64 pixel accumulator in sample order.
+// Each bounce kernel is persistent: a fixed grid pulls blocks of work off an
+// atomic counter until the input queue is drained, so the launch shape is fixed
+// (capturable in a hipGraph) while the live-ray count is known only on device.
+// A bounce does closest hit, shading, S shadow rays (traced inline: the scene is
+// small and L2-resident, the queue traffic is what costs HBM bandwidth), Russian
+// roulette and the next direction, then appends the surviving ray to the output
+// queue with one atomic per wave (ballot + mbcnt; a dispersive glass hit may
+// append several rays, handled with a wave prefix sum).
+//
+// Estimator semantics: the reference GPU estimator (code/Gpu/Opengl/
+// OpenglRenderer.cpp:783-1247: tracePath, tracePathSpectral,
+// tracePathSpectralSingle) evaluated with the reference CPU's material and
+// intersection rules (code/Renderer.cpp:468-949, code/Includes/Material.h).
+// The per-path arithmetic matches the oracle's orc_render_paths exactly.
+#include "pcr_device.h"
+#include "pcr_kernels.h"
+
+namespace pcr {
+
+// ------------------------------------------------------------ path state --
+struct PathState {
+    V3 o, d;
+    float lam0;
+    uint32_t meta;
+    float tp[4];
+    float rad[4];
+    uint64_t rng;
+    uint32_t path;
+};
+__device__ __forceinline__ int st_bounce(uint32_t m) { return (int)(m & 0xFFu); }
+__device__ __forceinline__ bool st_first(uint32_t m) { return (m >> 8) & 1u; }
+__device__ __forceinline__ int st_nch(uint32_t m) { return (int)((m >> 9) & 7u); }
+__device__ __forceinline__ int st_ch0(uint32_t m) { return (int)((m >> 12) & 3u); }
+__device__ __forceinline__ int st_slot(uint32_t m) { return (int)((m >> 16) & 0xFFu); }
+__device__ __forceinline__ uint32_t st_pack(int bounce, bool first, int nch, int ch0, int slot) {
+    return (uint32_t)bounce | ((uint32_t)first << 8) | ((uint32_t)nch << 9) | ((uint32_t)ch0 << 12) |
+           ((uint32_t)slot << 16);
+}
+
+__device__ __forceinline__ void st_load(const QueueView &q, uint32_t i, PathState &s) {
+    float4 a = q.od0[i], b = q.od1[i], t = q.tp[i], r = q.rad[i];
+    uint4 g = q.rng[i];
+    s.o = v3(a.x, a.y, a.z);
+    s.lam0 = a.w;
+    s.d = v3(b.x, b.y, b.z);
+    s.meta = __float_as_uint(b.w);
+    s.tp[0] = t.x; s.tp[1] = t.y; s.tp[2] = t.z; s.tp[3] = t.w;
+    s.rad[0] = r.x; s.rad[1] = r.y; s.rad[2] = r.z; s.rad[3] = r.w;
+    s.rng = (uint64_t)g.x | ((uint64_t)g.y << 32);
+    s.path = g.z;
+}
+__device__ __forceinline__ void st_store(const QueueView &q, uint32_t i, const PathState &s) {
+    q.od0[i] = make_float4(s.o.x, s.o.y, s.o.z, s.lam0);
+    q.od1[i] = make_float4(s.d.x, s.d.y, s.d.z, __uint_as_float(s.meta));
+    q.tp[i] = make_float4(s.tp[0], s.tp[1], s.tp[2], s.tp[3]);
+    q.rad[i] = make_float4(s.rad[0], s.rad[1], s.rad[2], s.rad[3]);
+    q.rng[i] = make_uint4((uint32_t)s.rng, (uint32_t)(s.rng >> 32), s.path, 0u);
+}
+
+// wavelength of hero channel c (Renderer.cpp:227-239)
+__device__ __forceinline__ float hero_lambda(float l0, int c, int N) {
+    if (c >= N || c == 0) return l0;
+    float stride = div_rn(300.f, (float)N);
+    float l = fadd(l0, fmul(stride, (float)c));
+    if (l > 700.f) l = fsub(l, 300.f);
+    return l;
+}
+
+// Pixel and sample of a job-local path index.
+__device__ __forceinline__ void path_pixel(const DJob &J, uint32_t path, int &px, int &py, int &s) {
+    uint32_t pix = path / (uint32_t)J.spp_pass;
+    uint32_t sl = path - pix * (uint32_t)J.spp_pass;
+    px = J.x0 + (int)(pix % (uint32_t)J.cw);
+    py = J.y0 + (int)(pix / (uint32_t)J.cw);
+    s = J.s_begin + (int)sl;
+}
+__device__ __forceinline__ uint64_t path_stream(const DJob &J, int px, int py, int s) {
+    uint64_t k = ((uint64_t)py * (uint64_t)J.W + (uint64_t)px) * (uint64_t)J.spp + (uint64_t)s;
+    return stream_seed(J.seed, k);
+}
+
+// Camera ray + the stream prefix draws (jitter, strata, hero wavelength).
+__device__ __forceinline__ void gen_path(const DJob &J, int slot, uint32_t path, PathState &ps) {
+    int px, py, s;
+    path_pixel(J, path, px, py, s);
+    ps.rng = path_stream(J, px, py, s);
+    float jx = 0.f, jy = 0.f;
+    if (J.flags & JF_JITTER) {
+        jx = fsub(rng_next(ps.rng), 0.5f);
+        jy = fsub(rng_next(ps.rng), 0.5f);
+    }
+    float x = fmul(fmul(fsub(div_rn(fmul(2.f, fadd(fadd((float)px, 0.5f), jx)), (float)J.W), 1.f), J.scale), J.aspect);
+    float y = fmul(-fsub(div_rn(fmul(2.f, fadd(fadd((float)py, 0.5f), jy)), (float)J.H), 1.f), J.scale);
+    ps.d = normalize(v3(x, y, -1.f));
+    ps.o = v3(J.S.cam.x, J.S.cam.y, J.S.cam.z);
+    if (J.flags & JF_STRATIFIED) {
+        rng_next(ps.rng);  // pr1, pr2: recomputed at the first diffuse vertex
+        rng_next(ps.rng);
+    }
+    int nch;
+    if (J.flags & JF_SPECTRAL) {
+        ps.lam0 = fadd(400.f, fmul(rng_next(ps.rng), 300.f));
+        nch = J.hero;
+    } else {
+        ps.lam0 = 0.f;
+        nch = 3;
+    }
+    for (int c = 0; c < 4; c++) {
+        ps.tp[c] = 1.f;
+        ps.rad[c] = 0.f;
+    }
+    ps.path = path;
+    ps.meta = st_pack(0, true, nch, 0, slot);
+}
+// The stratified first-bounce pair, recomputed from the stream prefix.
+__device__ __forceinline__ void strata_pair(const DJob &J, uint32_t path, float &pr1, float &pr2) {
+    int px, py, s;
+    path_pixel(J, path, px, py, s);
+    uint64_t r = path_stream(J, px, py, s);
+    if (J.flags & JF_JITTER) {
+        rng_next(r);
+        rng_next(r);
+    }
+    int sx = s % J.strata, sy = (s / J.strata) % J.strata;
+    pr1 = div_rn(fadd((float)sx, rng_next(r)), (float)J.strata);
+    pr2 = div_rn(fadd((float)sy, rng_next(r)), (float)J.strata);
+}
+
+__device__ __forceinline__ V3 fm_hemisphere(V3 N, float r1, float r2) {
+    V3 helper = fabsf(N.x) <= fabsf(N.y) ? v3(1.f, 0.f, 0.f) : v3(0.f, 1.f, 0.f);
+    V3 T = normalize(cross(N, helper));
+    V3 B = cross(N, T);
+    float r = sqrt_rn(r1);
+    float sn, cs;
+    fm_sincos_turns(r2, sn, cs);
+    float x = fmul(r, cs);
+    float y = fmul(r, sn);
+    float z = sqrt_rn(smax(0.f, fsub(fsub(1.f, fmul(x, x)), fmul(y, y))));
+    return add(add(mul(T, x), mul(B, y)), mul(N, z));
+}
+
+struct Bounce {
+    V3 dir, origin;
+};
+__device__ __forceinline__ Bounce dielectric(V3 rd, V3 N, V3 hit, bool entering, float ior, float u) {
+    float cosI = -dot(rd, N);
+    float n1 = entering ? 1.f : ior;
+    float n2 = entering ? ior : 1.f;
+    float eta = div_rn(n1, n2);
+    float sinT2 = fmul(fmul(eta, eta), fsub(1.f, fmul(cosI, cosI)));
+    Bounce b;
+    if (sinT2 >= 1.f) {
+        b.dir = add(rd, mul(N, fmul(2.f, cosI)));
+        b.origin = add(hit, mul(N, kEps3));
+        return b;
+    }
+    float F0 = div_rn(fsub(n1, n2), fadd(n1, n2));
+    F0 = fmul(F0, F0);
+    float m = fsub(1.f, cosI);
+    float m2 = fmul(m, m);
+    float F = fadd(F0, fmul(fsub(1.f, F0), fmul(fmul(m2, m2), m)));
+    if (u < F) {
+        b.dir = add(rd, mul(N, fmul(2.f, cosI)));
+        b.origin = add(hit, mul(N, kEps3));
+        return b;
+    }
+    float cosT = sqrt_rn(fsub(1.f, sinT2));
+    b.dir = add(mul(rd, eta), mul(N, fsub(fmul(eta, cosI), cosT)));
+    b.origin = sub(hit, mul(N, kEps3));
+    return b;
+}
+
+__device__ __forceinline__ void write_radiance(const DJob &J, const PathState &ps) {
+    int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
+    float *r = (float *)(J.radiance + ps.path);
+    if (ch0 == 0 && nch >= 3) {
+        J.radiance[ps.path] = make_float4(ps.rad[0], ps.rad[1], ps.rad[2], ps.rad[3]);
+    } else {
+        for (int c = 0; c < nch; c++) r[ch0 + c] = ps.rad[c];
+    }
+}
+
+struct BounceCounters {
+    uint32_t ext, shadow, emit;
+};
+
+// One bounce of one path.  Returns the number of output rays written to out[]
+// (0: terminated, 1: continues, up to 4: dispersion fork).
+template <int STACK>
+__device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathState &ps, PathState *out,
+                                           const LdsStack &st, BounceCounters &cnt) {
+    const int slot = st_slot(ps.meta);
+    const DJob &J = jobs[slot];
+    const DScene &S = J.S;
+    const bool spectral = (J.flags & JF_SPECTRAL) != 0;
+    const int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
+    const int bounce = st_bounce(ps.meta);
+    const bool first = st_first(ps.meta);
+    if (bounce >= J.depth) {
+        write_radiance(J, ps);
+        return 0;
+    }
+
+    float lam[4];
+    if (spectral) {
+        if (nch == 1 && ch0 > 0) lam[0] = hero_lambda(ps.lam0, ch0, J.hero);
+        else
+            for (int c = 0; c < 4; c++) lam[c] = hero_lambda(ps.lam0, c, J.hero);
+    }
+
+    cnt.ext++;
+    HitRec h = scene_closest<STACK>(S, ps.o, ps.d, 3.40282347e38f, st);
+    if (h.kind < 0) {
+        write_radiance(J, ps);
+        return 0;
+    }
+    V3 hit, N;
+    int mi;
+    hit_geometry(S, h, ps.o, ps.d, hit, N, mi);
+    bool entering = dot(ps.d, N) < 0.f;
+    if (!entering) N = mul(N, -1.f);
+    const DMat M = S.mat[mi];
+    if (M.flags & MF_EMISSIVE) {
+        cnt.emit++;
+        for (int c = 0; c < nch; c++) {
+            float le = spectral ? emissive_at(M, S.tabs, lam[c]) : (c == 0 ? M.emissive.x : (c == 1 ? M.emissive.y : M.emissive.z));
+            ps.rad[c] = fadd(ps.rad[c], fmul(ps.tp[c], le));
+        }
+        write_radiance(J, ps);
+        return 0;
+    }
+    float alb[4];
+    for (int c = 0; c < nch; c++)
+        alb[c] = spectral ? albedo_at(M, S.tabs, lam[c]) : (c == 0 ? M.albedo.x : (c == 1 ? M.albedo.y : M.albedo.z));
+
+    const int nb = bounce + 1;
+    if (M.flags & MF_METALLIC) {
+        float cosI = -dot(ps.d, N);
+        ps.d = add(ps.d, mul(N, fmul(2.f, cosI)));
+        ps.o = add(hit, mul(N, kEps3));
+        for (int c = 0; c < nch; c++) ps.tp[c] = fmul(ps.tp[c], alb[c]);
+        ps.meta = st_pack(nb, false, nch, ch0, slot);
+        if (nb >= J.depth) {
+            write_radiance(J, ps);
+            return 0;
+        }
+        out[0] = ps;
+        return 1;
+    }
+    if (M.flags & MF_TRANSPARENT) {
+        const float ior = M.albedo.w, cB = M.emissive.w;
+        if (spectral && cB != 0.f && nch > 1) {
+            // Dispersion fork: one single-wavelength sub-path per hero channel,
+            // each on its own stream derived from the parent's state.
+            int n = 0;
+            for (int c = 0; c < nch; c++) {
+                PathState ch = ps;
+                ch.rng = stream_seed(ps.rng, (uint64_t)(c + 1));
+                float iorc = cauchy_ior(ior, cB, lam[c]);
+                Bounce b = dielectric(ps.d, N, hit, entering, iorc, rng_next(ch.rng));
+                ch.o = b.origin;
+                ch.d = b.dir;
+                ch.tp[0] = fmul(ps.tp[c], alb[c]);
+                ch.rad[0] = ps.rad[c];
+                ch.tp[1] = ch.tp[2] = ch.tp[3] = 0.f;
+                ch.rad[1] = ch.rad[2] = ch.rad[3] = 0.f;
+                ch.meta = st_pack(nb, false, 1, ch0 + c, slot);
+                if (nb >= J.depth) {
+                    write_radiance(J, ch);
+                } else {
+                    out[n++] = ch;
+                }
+            }
+            return n;
+        }
+        float iorx = (spectral && nch == 1) ? cauchy_ior(ior, cB, lam[0]) : ior;
+        Bounce b = dielectric(ps.d, N, hit, entering, iorx, rng_next(ps.rng));
+        ps.d = b.dir;
+        ps.o = b.origin;
+        for (int c = 0; c < nch; c++) ps.tp[c] = fmul(ps.tp[c], alb[c]);
+        ps.meta = st_pack(nb, false, nch, ch0, slot);
+        if (nb >= J.depth) {
+            write_radiance(J, ps);
+            return 0;
+        }
+        out[0] = ps;
+        return 1;
+    }
+    // diffuse: next-event estimation (Renderer.cpp:565-653, OpenglRenderer.cpp:1150-1227)
+    if (S.total_area > 0.f) {
+        float dl[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < J.shadow; i++) {
+            int li = 0;
+            if (S.n_lights > 1) {
+                float target = fmul(rng_next(ps.rng), S.total_area);
+                float cum = 0.f;
+                for (int k = 0; k < S.n_lights; k++) {
+                    cum = fadd(cum, S.lights[k].area);
+                    if (target <= cum) { li = k; break; }
+                }
+            }
+            const DLight L = S.lights[li];
+            V3 sP, sN;
+            int lmat;
+            if (L.kind == 0) {
+                float ru = rng_next(ps.rng), rv = rng_next(ps.rng);
+                sP = add(add(xyz(L.o), mul(xyz(L.u), ru)), mul(xyz(L.v), rv));
+                sN = xyz(L.n);
+                lmat = __float_as_int(L.o.w);
+            } else {
+                float rt = fmul(rng_next(ps.rng), L.area);
+                int lo = 0, hi = L.count;
+                while (lo < hi) {  // std::lower_bound on the cumulative areas
+                    int mid = (lo + hi) >> 1;
+                    if (S.ltri[4 * (L.first + mid) + 1].w < rt) lo = mid + 1;
+                    else hi = mid;
+                }
+                int ti = L.first + (lo < L.count - 1 ? lo : L.count - 1);
+                float r1 = rng_next(ps.rng), r2 = rng_next(ps.rng);
+                if (fadd(r1, r2) > 1.f) {
+                    r1 = fsub(1.f, r1);
+                    r2 = fsub(1.f, r2);
+                }
+                const float4 *lt = S.ltri + 4 * ti;
+                float4 a = lt[0], e1 = lt[1], e2 = lt[2], fn = lt[3];
+                sP = add(add(xyz(a), mul(xyz(e1), r1)), mul(xyz(e2), r2));
+                sN = xyz(fn);
+                lmat = __float_as_int(a.w);
+            }
+            V3 Li = sub(sP, hit);
+            V3 wi = normalize(Li);
+            float cosT = smax(0.f, dot(wi, N));
+            float d2 = dot(Li, Li);
+            V3 so = cosT <= 0 ? sub(hit, mul(N, kEps3)) : add(hit, mul(N, kEps3));
+            cnt.shadow++;
+            HitRec sh = scene_closest<STACK>(S, so, wi, 3.40282347e38f, st);
+            bool occl = false;
+            if (sh.kind >= 0) {
+                V3 sp, sn;
+                int sm;
+                hit_geometry(S, sh, so, wi, sp, sn, sm);
+                V3 dd = sub(sp, so);
+                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(S.mat[sm].flags & MF_EMISSIVE);
+            }
+            if (!occl) {
+                float cosL = smax(0.f, dot(sN, mul(Li, -1.f)));
+                float G = div_rn(fmul(cosT, cosL), d2);
+                float w = 1.f;
+                if ((J.flags & JF_MIS) && cosL > 1e-6f) {
+                    float pl = div_rn(d2, fmul(cosL, S.total_area));
+                    float pb = div_rn(cosT, kPiF);
+                    w = div_rn(fmul(pl, pl), fadd(fmul(pl, pl), fmul(pb, pb)));
+                }
+                const DMat LM = S.mat[lmat];
+                for (int c = 0; c < nch; c++) {
+                    float le = spectral ? emissive_at(LM, S.tabs, lam[c])
+                                        : (c == 0 ? LM.emissive.x : (c == 1 ? LM.emissive.y : LM.emissive.z));
+                    dl[c] = fadd(dl[c], fmul(fmul(fmul(fmul(div_rn(alb[c], kPiF), le), G), S.total_area), w));
+                }
+            }
+        }
+        for (int c = 0; c < nch; c++) ps.rad[c] = fadd(ps.rad[c], fmul(ps.tp[c], div_rn(dl[c], (float)J.shadow)));
+    }
+    if ((J.flags & JF_RUSSIAN) && bounce >= 1) {
+        float a = spectral ? alb[0] : smax(smax(alb[0], alb[1]), alb[2]);
+        float p = smin(0.95f, smax(0.05f, a));
+        if (rng_next(ps.rng) > p) {
+            write_radiance(J, ps);
+            return 0;
+        }
+        for (int c = 0; c < nch; c++) ps.tp[c] = div_rn(ps.tp[c], p);
+    }
+    float r1, r2;
+    if ((J.flags & JF_STRATIFIED) && first) {
+        strata_pair(J, ps.path, r1, r2);
+    } else {
+        r1 = rng_next(ps.rng);
+        r2 = rng_next(ps.rng);
+    }
+    ps.d = fm_hemisphere(N, r1, r2);
+    ps.o = add(hit, mul(N, kEps3));
+    for (int c = 0; c < nch; c++) ps.tp[c] = fmul(ps.tp[c], alb[c]);
+    ps.meta = st_pack(nb, false, nch, ch0, slot);
+    if (nb >= J.depth) {
+        write_radiance(J, ps);
+        return 0;
+    }
+    out[0] = ps;
+    return 1;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Work item -> (job slot, job-local path index) for the generating bounce.
+__device__ __forceinline__ bool gen_lookup(const GenSeg *segs, int nseg, uint32_t item, int &slot, uint32_t &path) {
+    for (int k = 0; k < nseg; k++) {
+        GenSeg g = segs[k];
+        if (item < g.begin + g.count && item >= g.begin) {
+            slot = g.slot;
+            path = g.path0 + (item - g.begin);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Per-lane counters of the job slot the lane is currently processing; flushed
+// with atomics when the slot changes (only in mixed-job passes) and at exit.
+struct LaneCounters {
+    int slot = -1;
+    uint32_t paths = 0, ext = 0, shadow = 0, emit = 0;
+};
+__device__ __forceinline__ void flush_lane(const LaneCounters &c, unsigned long long *ctr) {
+    unsigned long long *p = ctr + 4 * c.slot;
+    if (c.paths) atomicAdd(p + 0, (unsigned long long)c.paths);
+    if (c.ext) atomicAdd(p + 1, (unsigned long long)c.ext);
+    if (c.shadow) atomicAdd(p + 2, (unsigned long long)c.shadow);
+    if (c.emit) atomicAdd(p + 3, (unsigned long long)c.emit);
+}
+
+template <bool FIRST, int STACK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
+    __shared__ int32_t s_idx[STACK * BLOCK];
+    __shared__ float s_tn[STACK * BLOCK];
+    __shared__ uint32_t s_base;
+    LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
+    // An overflowing append still counts, so never read past the queue.
+    const uint32_t n_in = FIRST ? a.n_gen : min(*a.count_in, a.capacity);
+    LaneCounters lc;
+    const uint32_t lane = lane_id();
+    for (;;) {
+        if (threadIdx.x == 0) s_base = atomicAdd(a.work, (uint32_t)BLOCK);
+        __syncthreads();
+        const uint32_t base = s_base;
+        __syncthreads();
+        if (base >= n_in) break;
+        const uint32_t i = base + threadIdx.x;
+        PathState outs[4];
+        int nout = 0;
+        if (i < n_in) {
+            PathState ps;
+            bool ok = true;
+            int slot = 0;
+            if (FIRST) {
+                uint32_t path;
+                ok = gen_lookup(a.segs, a.n_segs, i, slot, path);
+                if (ok) gen_path(a.jobs[slot], slot, path, ps);
+            } else {
+                st_load(a.in, i, ps);
+                slot = st_slot(ps.meta);
+            }
+            if (ok) {
+                if (slot != lc.slot) {
+                    if (lc.slot >= 0) flush_lane(lc, a.counters);
+                    lc = LaneCounters{};
+                    lc.slot = slot;
+                }
+                BounceCounters cnt{0u, 0u, 0u};
+                nout = bounce_path<STACK>(a.jobs, ps, outs, st, cnt);
+                lc.paths += FIRST ? 1u : 0u;
+                lc.ext += cnt.ext;
+                lc.shadow += cnt.shadow;
+                lc.emit += cnt.emit;
+            }
+        }
+        // compaction: a wave reserves its output range with one atomic
+        const uint64_t multi = __ballot(nout > 1);
+        uint32_t off, total;
+        if (multi == 0) {
+            const uint64_t m = __ballot(nout == 1);
+            off = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            total = (uint32_t)__popcll(m);
+        } else {
+            uint32_t x = (uint32_t)nout;
+            for (int dd = 1; dd < 64; dd <<= 1) {
+                uint32_t y = __shfl_up(x, dd, 64);
+                if (lane >= (uint32_t)dd) x += y;
+            }
+            total = __shfl(x, 63, 64);
+            off = x - (uint32_t)nout;
+        }
+        uint32_t wbase = 0;
+        if (total) {
+            if (lane == 0) wbase = atomicAdd(a.count_out, total);
+            wbase = __shfl(wbase, 0, 64);
+        }
+        for (int k = 0; k < nout; k++) {
+            uint32_t dst = wbase + off + (uint32_t)k;
+            if (dst < a.capacity) st_store(a.out, dst, outs[k]);
+            else atomicOr(a.overflow, 1u);
+        }
+    }
+    // counters: one atomic per wave when the wave worked on a single job
+    int key = lc.slot < 0 ? 0x7fffffff : lc.slot;
+    int mn = key;
+    for (int dd = 32; dd >= 1; dd >>= 1) mn = min(mn, __shfl_xor(mn, dd, 64));
+    const bool uniform = __ballot(key != mn && key != 0x7fffffff) == 0;
+    if (uniform) {
+        if (mn != 0x7fffffff) {
+            uint32_t p = lc.paths, e = lc.ext, s = lc.shadow, m = lc.emit;
+            for (int dd = 32; dd >= 1; dd >>= 1) {
+                p += __shfl_down(p, dd, 64);
+                e += __shfl_down(e, dd, 64);
+                s += __shfl_down(s, dd, 64);
+                m += __shfl_down(m, dd, 64);
+            }
+            if (lane == 0) {
+                LaneCounters w;
+                w.slot = mn;
+                w.paths = p;
+                w.ext = e;
+                w.shadow = s;
+                w.emit = m;
+                flush_lane(w, a.counters);
+            }
+        }
+    } else if (lc.slot >= 0) {
+        flush_lane(lc, a.counters);
+    }
+}
+
+// Per pixel of each job in the pass: radiance -> XYZ (singleLambdaXYZ,
+// CIE.h:140-159, averaged over the N hero channels) or RGB, added to the FP64
+// accumulator in sample order.
+__global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    int slot = -1;
+    uint32_t pix = 0;
+    for (int k = 0; k < a.n_segs; k++) {
+        ResolveSeg g = a.segs[k];
+        if (item >= g.begin && item < g.begin + g.count) {
+            slot = g.slot;
+            pix = item - g.begin;
+            break;
+        }
+    }
+    if (slot < 0) return;
+    const DJob &J = a.jobs[slot];
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    double *dst = J.accum + 3 * (size_t)pix;
+    acc0 = dst[0];
+    acc1 = dst[1];
+    acc2 = dst[2];
+    const bool spectral = (J.flags & JF_SPECTRAL) != 0;
+    const int N = J.hero;
+    const float sc = div_rn(300.f, kYbarIntegral);
+    for (int sl = 0; sl < J.spp_pass; sl++) {
+        uint32_t path = pix * (uint32_t)J.spp_pass + (uint32_t)sl;
+        float4 r = J.radiance[path];
+        float o0, o1, o2;
+        if (spectral) {
+            int px, py, s;
+            path_pixel(J, path, px, py, s);
+            uint64_t rng = path_stream(J, px, py, s);
+            if (J.flags & JF_JITTER) { rng_next(rng); rng_next(rng); }
+            if (J.flags & JF_STRATIFIED) { rng_next(rng); rng_next(rng); }
+            float l0 = fadd(400.f, fmul(rng_next(rng), 300.f));
+            float rr[4] = {r.x, r.y, r.z, r.w};
+            float X = 0.f, Y = 0.f, Z = 0.f;
+            for (int c = 0; c < N; c++) {
+                V3 cmf = fm_cmf(hero_lambda(l0, c, N));
+                X = fadd(X, fmul(fmul(rr[c], cmf.x), sc));
+                Y = fadd(Y, fmul(fmul(rr[c], cmf.y), sc));
+                Z = fadd(Z, fmul(fmul(rr[c], cmf.z), sc));
+            }
+            o0 = div_rn(X, (float)N);
+            o1 = div_rn(Y, (float)N);
+            o2 = div_rn(Z, (float)N);
+        } else {
+            o0 = r.x;
+            o1 = r.y;
+            o2 = r.z;
+        }
+        acc0 += (double)o0;
+        acc1 += (double)o1;
+        acc2 += (double)o2;
+    }
+    dst[0] = acc0;
+    dst[1] = acc1;
+    dst[2] = acc2;
+}
+
+// Closest hit for a fixed ray set (parity of hit ids with the reference).
+template <int STACK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_intersect(IntersectArgs a) {
+    __shared__ int32_t s_idx[STACK * BLOCK];
+    __shared__ float s_tn[STACK * BLOCK];
+    LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const float *r = a.rays + 6 * (size_t)i;
+    V3 o = v3(r[0], r[1], r[2]), d = v3(r[3], r[4], r[5]);
+    HitRec h = scene_closest<STACK>(a.S, o, d, a.tmax, st);
+    int kind = h.kind, prim = h.prim, mat = -1;
+    V3 P = v3(0.f, 0.f, 0.f), N = v3(0.f, 0.f, 0.f);
+    if (kind >= 0) {
+        hit_geometry(a.S, h, o, d, P, N, mat);
+        if (kind == 2 && a.tri_orig) prim = a.tri_orig[prim];
+    }
+    a.t[i] = kind >= 0 ? h.t : __builtin_huge_valf();
+    a.kind[i] = kind;
+    a.prim[i] = kind >= 0 ? prim : -1;
+    a.mat[i] = mat;
+    if (a.hit) {
+        a.hit[3 * i] = P.x;
+        a.hit[3 * i + 1] = P.y;
+        a.hit[3 * i + 2] = P.z;
+    }
+    if (a.normal) {
+        a.normal[3 * i] = N.x;
+        a.normal[3 * i + 1] = N.y;
+        a.normal[3 * i + 2] = N.z;
+    }
+}
+
+// Zeroes the per-pass work/queue counters (a kernel, not a memset, so the
+// pass can be captured in a hipGraph).
+__global__ void k_reset(uint32_t *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Advances the pass index of every active job slot (graph replay state).
+__global__ void k_advance(DJob *jobs, const int32_t *slots, int n) {
+    int i = threadIdx.x;
+    if (i < n) {
+        DJob &J = jobs[slots[i]];
+        J.s_begin += J.spp_pass;
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+template <bool FIRST, int STACK, int BLOCK>
+static hipError_t launch_bounce_t(const BounceArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_bounce<FIRST, STACK, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+template <int STACK>
+static hipError_t launch_bounce_s(const BounceArgs &a, bool first, int grid, int block, hipStream_t s) {
+    if (block == 64) return first ? launch_bounce_t<true, STACK, 64>(a, grid, s) : launch_bounce_t<false, STACK, 64>(a, grid, s);
+    if (block == 128) return first ? launch_bounce_t<true, STACK, 128>(a, grid, s) : launch_bounce_t<false, STACK, 128>(a, grid, s);
+    return first ? launch_bounce_t<true, STACK, 256>(a, grid, s) : launch_bounce_t<false, STACK, 256>(a, grid, s);
+}
+
+}  // namespace pcr
+
+extern "C" hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block,
+                                        hipStream_t s) {
+    using namespace pcr;
+    if (stack <= 8) return launch_bounce_s<8>(*a, first != 0, grid, block, s);
+    if (stack <= 16) return launch_bounce_s<16>(*a, first != 0, grid, block, s);
+    if (stack <= 32) return launch_bounce_s<32>(*a, first != 0, grid, block == 256 ? 128 : block, s);
+    return launch_bounce_s<64>(*a, first != 0, grid, 64, s);
+}
+extern "C" hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, uint32_t n_items, hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(pcr::k_resolve, dim3((n_items + 255) / 256), dim3(256), 0, s, *a);
+    return hipGetLastError();
+}
+extern "C" hipError_t pcr_launch_intersect(const pcr::IntersectArgs *a, int stack, hipStream_t s) {
+    using namespace pcr;
+    if (a->n == 0) return hipSuccess;
+    if (stack <= 16) hipLaunchKernelGGL((k_intersect<16, 128>), dim3((a->n + 127) / 128), dim3(128), 0, s, *a);
+    else if (stack <= 32) hipLaunchKernelGGL((k_intersect<32, 128>), dim3((a->n + 127) / 128), dim3(128), 0, s, *a);
+    else hipLaunchKernelGGL((k_intersect<64, 64>), dim3((a->n + 63) / 64), dim3(64), 0, s, *a);
+    return hipGetLastError();
+}
+extern "C" hipError_t pcr_launch_reset(uint32_t *p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(pcr::k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+extern "C" hipError_t pcr_launch_advance(pcr::DJob *jobs, const int32_t *slots, int n, hipStream_t s) {
+    hipLaunchKernelGGL(pcr::k_advance, dim3(1), dim3(64), 0, s, jobs, slots, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,176 @@
+// HipRenderer.cpp -- see HipRenderer.h.
+#include "HipRenderer.h"
+
+#include <algorithm>
+#include <cctype>
+#include <ctime>
+#include <filesystem>
+#include <iostream>
+#include <thread>
+
+#include "output.h"
+
+#ifndef PCR_BINARY_NAME
+#define PCR_BINARY_NAME "pcr-hip"
+#endif
+
+namespace pcr {
+
+namespace {
+
+// "YYYYMMDD-HHMMSS-<ZONE>" in local time (Renderer.cpp:79-105).
+std::string timestamp() {
+    std::time_t now = std::time(nullptr);
+    std::tm tm{};
+    localtime_r(&now, &tm);
+    char buf[64], zone[64];
+    std::strftime(buf, sizeof(buf), "%Y%m%d-%H%M%S", &tm);
+    std::strftime(zone, sizeof(zone), "%Z", &tm);
+    std::string z = zone;
+    if (z.find(' ') != std::string::npos) {  // long zone names -> initials
+        std::string a;
+        bool ws = true;
+        for (char c : z) {
+            if (c == ' ') { ws = true; continue; }
+            if (ws && std::isalpha((unsigned char)c)) a += (char)std::toupper((unsigned char)c);
+            ws = false;
+        }
+        z = a;
+    }
+    return std::string(buf) + "-" + z;
+}
+
+}  // namespace
+
+HipRenderer::HipRenderer(int width, int height, int depth, int samples, int shadowSamples, void *)
+    : _width(width), _height(height), _maxDepth(depth), _samples(samples), _shadowSamples(shadowSamples) {}
+
+HipRenderer::~HipRenderer() {
+    if (_ctx) pcrhip_destroy(_ctx);
+}
+
+void HipRenderer::render(const Scene &scene, std::chrono::steady_clock::time_point start,
+                         const std::string &outputDir) {
+    lastOutputPath.clear();
+    if (useDenoise) std::cerr << "warning: --denoise (bilateral) is not provided by the HIP backend; skipping.\n";
+    if (useOIDN) std::cerr << "warning: --oidn is not provided by the HIP backend; skipping.\n";
+    if (useAdaptive) std::cerr << "warning: --adaptive is not provided by the HIP backend; rendering all samples.\n";
+    if (!_ctx && pcrhip_create(device, &_ctx) != PCRHIP_OK) {
+        std::cerr << "HIP backend: " << pcrhip_last_error() << std::endl;
+        return;
+    }
+    PackedScene packed;
+    pack_scene(scene, packed);
+    pcrhip_scene *ds = nullptr;
+    if (pcrhip_scene_upload(_ctx, &packed.desc, &ds) != PCRHIP_OK) {
+        std::cerr << "HIP backend: scene upload failed: " << pcrhip_last_error() << std::endl;
+        return;
+    }
+    pcrhip_job_desc jd{};
+    jd.width = _width;
+    jd.height = _height;
+    jd.depth = _maxDepth;
+    int aa = std::max(1, aaSamples);
+    jd.spp = _samples * aa;
+    jd.shadow = _shadowSamples;
+    jd.hero = std::clamp(heroSamples, 1, 4);
+    uint32_t fl = 0;
+    if (useSpectral) fl |= PCRHIP_SPECTRAL;
+    if (useMIS) fl |= PCRHIP_MIS;
+    if (useRussian) fl |= PCRHIP_RUSSIAN;
+    if (useStratified) fl |= PCRHIP_STRATIFIED;
+    if (aa > 1) fl |= PCRHIP_JITTER;
+    jd.flags = fl;
+    // The reference GPU backends seed from the clock (OpenglRenderer.cpp:1995-1997).
+    jd.seed = seed ? seed
+                   : (uint64_t)std::chrono::duration_cast<std::chrono::milliseconds>(
+                         std::chrono::steady_clock::now().time_since_epoch())
+                         .count();
+    pcrhip_job *job = nullptr;
+    if (pcrhip_job_submit(_ctx, ds, &jd, &job) != PCRHIP_OK) {
+        std::cerr << "HIP backend: " << pcrhip_last_error() << std::endl;
+        pcrhip_scene_release(ds);
+        return;
+    }
+    bool cancelled = false;
+    for (;;) {
+        int rc = pcrhip_job_wait(job, 20);
+        int done = 0, total = 0;
+        pcrhip_job_progress(job, &done, &total);
+        if (progressRows) progressRows->store(done, std::memory_order_relaxed);
+        if (rc != PCRHIP_ETIMEOUT) break;
+        if (cancelRequested && cancelRequested->load(std::memory_order_relaxed) && !cancelled) {
+            pcrhip_job_cancel(job);
+            cancelled = true;
+        }
+    }
+    std::vector<float> mean((size_t)_width * _height * 3);
+    int rc = pcrhip_job_result(job, mean.data(), mean.size(), &lastStats);
+    std::string err = rc ? pcrhip_last_error() : "";
+    pcrhip_job_release(job);
+    pcrhip_scene_release(ds);
+    if (rc == PCRHIP_ECANCELLED || (cancelRequested && cancelRequested->load(std::memory_order_relaxed))) {
+        std::cout << "Render cancelled before write." << std::endl;
+        return;
+    }
+    if (rc != PCRHIP_OK) {
+        std::cerr << "HIP backend: render failed: " << err << std::endl;
+        return;
+    }
+    lastFrame.assign((size_t)_width * _height, Vec3{});
+    for (size_t i = 0; i < lastFrame.size(); i++) {
+        Vec3 c{mean[3 * i], mean[3 * i + 1], mean[3 * i + 2]};
+        lastFrame[i] = useSpectral ? xyz_to_linear_srgb(c) : c;
+    }
+    if (onPartialFrame) onPartialFrame(lastFrame, _width, _height);
+    std::vector<uint8_t> rgb = frame_to_rgb8(lastFrame, useACES);
+
+    auto end = std::chrono::steady_clock::now();
+    long long ms = std::chrono::duration_cast<std::chrono::milliseconds>(end - start).count();
+    std::cout << "Render took " << ms << " ms" << std::endl;
+    std::string ts = timestamp();
+    std::string name = scene.name + "-" + scene.version + "-" + ts + "-d" + std::to_string(_maxDepth) + "-s" +
+                       std::to_string(_samples) + "-S" + std::to_string(_shadowSamples) + "-w" + std::to_string(_width);
+    if (_width != _height) name += "-h" + std::to_string(_height);
+    if (useSpectral) name += "-spectral";
+    if (useACES) name += "-aces";
+    name += "-t" + std::to_string(ms) + "-gpu.png";
+    std::filesystem::path out = std::filesystem::path(outputDir) / name;
+    std::error_code ec;
+    std::filesystem::create_directories(out.parent_path(), ec);
+    std::vector<std::pair<std::string, std::string>> text = {
+        {"Software", PCR_BINARY_NAME},
+        {"Backend", "GPU HIP gfx950 wavefront"},
+        {"Scene", scene.name},
+        {"SceneVersion", scene.version},
+        {"CreationTime", ts},
+        {"Depth", std::to_string(_maxDepth)},
+        {"Samples", std::to_string(_samples)},
+        {"ShadowSamples", std::to_string(_shadowSamples)},
+        {"Width", std::to_string(_width)},
+        {"Height", std::to_string(_height)},
+        {"RenderTimeMs", std::to_string(ms)},
+        {"Denoise", useDenoise ? "1" : "0"},
+        {"MIS", useMIS ? "1" : "0"},
+        {"Russian", useRussian ? "1" : "0"},
+        {"Stratified", useStratified ? "1" : "0"},
+        {"Tonemap", useACES ? "ACES" : "Reinhard"},
+        {"AASamples", std::to_string(aaSamples)},
+        {"Adaptive", useAdaptive ? "1" : "0"},
+        {"OIDN", useOIDN ? "1" : "0"},
+        {"Spectral", useSpectral ? "1" : "0"},
+        {"WavefrontMode", "persistent-fused"},
+        {"Seed", std::to_string(jd.seed)},
+    };
+    if (useSpectral) text.push_back({"HeroSamples", std::to_string(std::clamp(heroSamples, 1, 4))});
+    std::vector<uint8_t> png;
+    std::string perr;
+    if (!encode_png(rgb, _width, _height, text, png, &perr) || !write_file(out.string(), png, &perr)) {
+        std::cerr << "PNG output failed: " << perr << std::endl;
+        return;
+    }
+    std::cout << "Wrote " << out << std::endl;
+    lastOutputPath = out.string();
+}
+
+}  // namespace pcr

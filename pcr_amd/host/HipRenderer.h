@@ -1,0 +1,70 @@
+// HipRenderer.h -- pcr's renderer-backend interface over the MI355X backend.
+//
+// Same public surface as the reference backends (code/Includes/Renderer.h:28-161,
+// code/Gpu/Opengl/OpenglRenderer.h:29-117, code/Gpu/Metal/MetalRenderer.h:27-106):
+// constructor (width, height, depth, samples, shadowSamples[, shared context]),
+// the progressRows / cancelRequested / onPartialFrame hooks, the technique
+// knobs, render(scene, start, outputDir) and lastOutputPath.  `samples` is
+// paths per pixel as in the GPU backends; with aaSamples > 1 every pixel gets
+// aaSamples * samples jittered paths.  The frame goes to a PNG with the
+// reference's filename contract and tEXt keys (Renderer.cpp:387-465,
+// OpenglRenderer.cpp:2335-2404).
+#pragma once
+
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/pcrhip.h"
+#include "scene.h"
+
+namespace pcr {
+
+class HipRenderer {
+public:
+    HipRenderer(int width, int height, int depth, int samples, int shadowSamples, void *sharedContext = nullptr);
+    ~HipRenderer();
+    HipRenderer(const HipRenderer &) = delete;
+    HipRenderer &operator=(const HipRenderer &) = delete;
+
+    std::atomic<int> *progressRows = nullptr;
+    std::atomic<bool> *cancelRequested = nullptr;
+    // Called with the final linear frame (the reference GPU backends never call
+    // it; kept for interface compatibility).
+    std::function<void(const std::vector<Vec3> &, int, int)> onPartialFrame;
+
+    bool useDenoise = false;  // bilateral post-filter: not provided by this backend (warns)
+    bool useMIS = false;
+    bool useRussian = false;
+    bool useStratified = false;
+    bool useACES = false;
+    int aaSamples = 1;
+    bool useAdaptive = false;  // adaptive AA: not provided by this backend (warns)
+    bool useOIDN = false;      // OIDN: not provided by this backend (warns)
+    bool useSpectral = false;
+    int heroSamples = 4;
+    int threadgroupX = 8, threadgroupY = 8;  // accepted for interface compatibility
+    bool useWavefront = true;
+    bool wavefrontMultiSample = false;
+    bool spectralFork = true;
+
+    // Extensions: NumGen seed of the render (0: derived from the clock, as the
+    // reference GPU backends do) and the device ordinal.
+    uint64_t seed = 0;
+    int device = 0;
+
+    void render(const Scene &scene, std::chrono::steady_clock::time_point start, const std::string &outputDir);
+
+    std::string lastOutputPath;
+    std::vector<Vec3> lastFrame;  // linear sRGB of the last render
+    pcrhip_stats lastStats{};
+
+private:
+    int _width, _height, _maxDepth, _samples, _shadowSamples;
+    pcrhip_ctx *_ctx = nullptr;
+};
+
+}  // namespace pcr

@@ -1,0 +1,111 @@
+// capi.cpp -- C entry points of the host library (libpcrhost.so) used by the
+// Python harness: scene ingestion into a pcrhip_scene_desc, the BVH builder and
+// the spectral fit, and PNG/tone-map helpers.  The device boundary itself is
+// include/pcrhip.h.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "output.h"
+#include "scene.h"
+
+namespace {
+thread_local std::string g_host_err;
+
+struct HostScene {
+    pcr::Scene scene;
+    pcr::PackedScene packed;
+};
+}  // namespace
+
+extern "C" {
+
+const char *pcrhost_last_error(void) { return g_host_err.c_str(); }
+
+// Loads and ingests a scene file; NULL on error (see pcrhost_last_error).
+void *pcrhost_scene_load(const char *path) {
+    try {
+        auto h = std::make_unique<HostScene>();
+        h->scene = pcr::load_scene(path);
+        pcr::pack_scene(h->scene, h->packed);
+        return h.release();
+    } catch (const std::exception &e) {
+        g_host_err = e.what();
+        return nullptr;
+    }
+}
+void pcrhost_scene_free(void *h) { delete (HostScene *)h; }
+const pcrhip_scene_desc *pcrhost_scene_desc(void *h) { return &((HostScene *)h)->packed.desc; }
+const char *pcrhost_scene_name(void *h) { return ((HostScene *)h)->scene.name.c_str(); }
+int pcrhost_scene_bvh_depth(void *h) { return pcr::bvh_depth(((HostScene *)h)->scene.bvh); }
+
+// BVH over n triangles (9 floats each).  nodes: room for 2n-1 nodes of 9 int32;
+// perm: n.  Returns the node count.
+int pcrhost_bvh_build(const float *tris, int n, int32_t *nodes, int32_t *perm) {
+    std::vector<pcr::Triangle> t((size_t)n);
+    for (int i = 0; i < n; i++) {
+        const float *p = tris + 9 * i;
+        t[i].v0 = pcr::Vec3{p[0], p[1], p[2]};
+        t[i].v1 = pcr::Vec3{p[3], p[4], p[5]};
+        t[i].v2 = pcr::Vec3{p[6], p[7], p[8]};
+        t[i].orig = i;
+    }
+    auto nn = pcr::build_bvh(t);
+    for (size_t i = 0; i < nn.size(); i++) {
+        std::memcpy(nodes + 9 * i, nn[i].bmin, 12);
+        std::memcpy(nodes + 9 * i + 3, nn[i].bmax, 12);
+        nodes[9 * i + 6] = nn[i].left;
+        nodes[9 * i + 7] = nn[i].right;
+        nodes[9 * i + 8] = nn[i].count;
+    }
+    for (int i = 0; i < n; i++) perm[i] = t[i].orig;
+    return (int)nn.size();
+}
+
+void pcrhost_fit_sigmoid(const float rgb[3], float out[4]) {
+    pcr::SigmoidFit f = pcr::fit_sigmoid(pcr::Vec3{rgb[0], rgb[1], rgb[2]});
+    out[0] = f.c0;
+    out[1] = f.c1;
+    out[2] = f.c2;
+    out[3] = f.scale;
+}
+
+int pcrhost_load_spd(const char *path, float out61[61]) {
+    std::array<float, 61> a{};
+    std::string err;
+    if (!pcr::load_spd_file(path, a, &err)) {
+        g_host_err = err;
+        return -1;
+    }
+    std::memcpy(out61, a.data(), sizeof(float) * 61);
+    return 0;
+}
+
+// linear frame (w*h*3) -> tone-mapped RGB8 (w*h*3)
+void pcrhost_tonemap(const float *frame, int n_pixels, int aces, uint8_t *rgb8) {
+    std::vector<pcr::Vec3> f((size_t)n_pixels);
+    for (int i = 0; i < n_pixels; i++) f[i] = pcr::Vec3{frame[3 * i], frame[3 * i + 1], frame[3 * i + 2]};
+    auto out = pcr::frame_to_rgb8(f, aces != 0);
+    std::memcpy(rgb8, out.data(), out.size());
+}
+
+// Writes an RGB8 PNG with tEXt chunks given as "key\0value\0key\0value\0\0".
+int pcrhost_write_png(const char *path, const uint8_t *rgb8, int w, int h, const char *kv) {
+    std::vector<std::pair<std::string, std::string>> text;
+    while (kv && *kv) {
+        std::string k = kv;
+        kv += k.size() + 1;
+        std::string v = kv;
+        kv += v.size() + 1;
+        text.emplace_back(k, v);
+    }
+    std::vector<uint8_t> rgb(rgb8, rgb8 + (size_t)w * h * 3), png;
+    std::string err;
+    if (!pcr::encode_png(rgb, w, h, text, png, &err) || !pcr::write_file(path, png, &err)) {
+        g_host_err = err;
+        return -1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// cli.cpp -- pcr-hip: the command-line renderer on the MI355X backend.
+//
+// Keeps the reference CLI's flags and behaviour (code/Cli/Main.cpp:75-481):
+// scene discovery over --scenes-dir, $PWD/Scenes and <binary-dir>/Scenes
+// (first file declaring a name wins), -d/-s/-S/-w/--height, -o (default
+// $PWD/Image), technique flags, --hero-samples, --seed, --list-scenes, and the
+// exit codes (1 for an unknown scene or a load error).  Render time is
+// measured from process start as the reference does.  --lut / --lut-file
+// (optional RGB->spectrum lookup-table cache) are not provided.
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "HipRenderer.h"
+#include "json.h"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+struct Found {
+    std::string name, version, path;
+};
+
+fs::path exe_dir() {
+    std::error_code ec;
+    fs::path p = fs::read_symlink("/proc/self/exe", ec);
+    return ec ? fs::current_path() : p.parent_path();
+}
+
+std::vector<Found> discover(const std::vector<std::string> &extra) {
+    std::vector<fs::path> dirs(extra.begin(), extra.end());
+    dirs.push_back(fs::current_path() / "Scenes");
+    dirs.push_back(exe_dir() / "Scenes");
+    std::vector<Found> out;
+    std::map<std::string, bool> seen;
+    for (const auto &d : dirs) {
+        std::error_code ec;
+        if (!fs::is_directory(d, ec)) continue;
+        std::vector<fs::path> files;
+        for (const auto &e : fs::directory_iterator(d, ec))
+            if (e.is_regular_file() && e.path().extension() == ".json") files.push_back(e.path());
+        std::sort(files.begin(), files.end());
+        for (const auto &f : files) {
+            try {
+                std::ifstream in(f);
+                std::stringstream ss;
+                ss << in.rdbuf();
+                pcr::json::Value j = pcr::json::parse(ss.str());
+                const auto *n = j.find("name");
+                const auto *v = j.find("version");
+                if (!n || !n->is_string() || !j.find("primitives")) continue;
+                if (seen[n->str]) continue;
+                seen[n->str] = true;
+                out.push_back(Found{n->str, v && v->is_string() ? v->str : "", f.string()});
+            } catch (const std::exception &e) {
+                std::cerr << "warning: Skipping " << f.string() << ": " << e.what() << "\n";
+            }
+        }
+    }
+    return out;
+}
+
+[[noreturn]] void usage(int code) {
+    std::cout << "pcr-hip - GPU path tracer (MI355X, headless)\n"
+                 "  --scene NAME            scene to render (default cornell)\n"
+                 "  -d,--depth N            max ray bounces (4)\n"
+                 "  -s,--samples N          paths per pixel (16)\n"
+                 "  -S,--shadow N           shadow rays per diffuse hit (4)\n"
+                 "  -w,--width N            output width (720)\n"
+                 "  --height N              output height (default: width)\n"
+                 "  -o,--output DIR         output directory (default $PWD/Image)\n"
+                 "  --scenes-dir DIR        extra scene directory (repeatable)\n"
+                 "  --list-scenes           list discovered scenes and exit\n"
+                 "  --mis --russian --stratified --aces --spectral\n"
+                 "  --aa [--aa-samples N]   jittered primary rays (N x samples paths per pixel)\n"
+                 "  --adaptive --denoise --oidn   accepted; not provided by this backend\n"
+                 "  --hero-samples N        hero wavelengths 1..4 (4)\n"
+                 "  --seed N                NumGen seed (0: from the clock)\n"
+                 "  --device N              GPU ordinal (0)\n";
+    std::exit(code);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    auto start = std::chrono::steady_clock::now();
+    int depth = 4, samples = 16, shadow = 4, width = 720, height = -1, aaSamples = 4, hero = 4, device = 0;
+    std::string scene = "cornell", outputDir;
+    std::vector<std::string> dirs;
+    bool list = false, denoise = false, mis = false, russian = false, strat = false, aces = false, aa = false,
+         adaptive = false, oidn = false, spectral = false;
+    uint64_t seed = 0;
+    // "--flag=value" is split into "--flag" "value" (CLI11 accepts both forms)
+    std::vector<std::string> args;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        auto eq = a.find('=');
+        if (a.rfind("--", 0) == 0 && eq != std::string::npos) {
+            args.push_back(a.substr(0, eq));
+            args.push_back(a.substr(eq + 1));
+        } else {
+            args.push_back(a);
+        }
+    }
+    const int n = (int)args.size();
+    auto need = [&](int &i) -> std::string {
+        if (i + 1 >= n) {
+            std::cerr << "missing value for " << args[i] << "\n";
+            std::exit(109);
+        }
+        return args[++i];
+    };
+    auto posint = [&](int &i) {
+        std::string a = args[i];
+        std::string v = need(i);
+        char *end = nullptr;
+        long x = std::strtol(v.c_str(), &end, 10);
+        if (*end || x <= 0) {
+            std::cerr << a << ": value must be a positive number\n";
+            std::exit(105);
+        }
+        return (int)x;
+    };
+    for (int i = 0; i < n; i++) {
+        const std::string a = args[i];
+        if (a == "-h" || a == "--help") usage(0);
+        else if (a == "--scene") scene = need(i);
+        else if (a == "-d" || a == "--depth") depth = posint(i);
+        else if (a == "-s" || a == "--samples") samples = posint(i);
+        else if (a == "-S" || a == "--shadow") shadow = posint(i);
+        else if (a == "-w" || a == "--width") width = posint(i);
+        else if (a == "--height") height = posint(i);
+        else if (a == "-o" || a == "--output") outputDir = need(i);
+        else if (a == "--scenes-dir") dirs.push_back(need(i));
+        else if (a == "--list-scenes") list = true;
+        else if (a == "--denoise") denoise = true;
+        else if (a == "--mis") mis = true;
+        else if (a == "--russian") russian = true;
+        else if (a == "--stratified") strat = true;
+        else if (a == "--aces") aces = true;
+        else if (a == "--aa") aa = true;
+        else if (a == "--aa-samples") aaSamples = posint(i);
+        else if (a == "--adaptive") adaptive = true;
+        else if (a == "--oidn") oidn = true;
+        else if (a == "--spectral") spectral = true;
+        else if (a == "--hero-samples") {
+            hero = posint(i);
+            if (hero > 4) {
+                std::cerr << "--hero-samples: value must be in [1 - 4]\n";
+                return 105;
+            }
+        } else if (a == "--seed") seed = std::strtoull(need(i).c_str(), nullptr, 10);
+        else if (a == "--device") device = std::atoi(need(i).c_str());
+        else if (a == "--tz" || a == "--timezone") setenv("TZ", need(i).c_str(), 1);
+        else if (a == "--lut" || a == "--lut-file") {
+            std::cerr << a << ": the RGB-to-spectrum lookup-table cache is not provided by this build\n";
+            return 1;
+        } else if (a == "--threadgroup-x" || a == "--threadgroup-y") need(i);
+        else if (a == "--wavefront" || a == "--no-wavefront" || a == "--wavefront-multi-sample" ||
+                 a == "--spectral-terminate") {
+        } else {
+            std::cerr << "The following argument was not expected: " << a << "\n";
+            return 109;
+        }
+    }
+    if (height < 0) height = width;
+    if (outputDir.empty()) outputDir = (fs::current_path() / "Image").string();
+    auto reg = discover(dirs);
+    if (list) {
+        std::cout << "Available scenes:\n";
+        for (const auto &f : reg) std::cout << "  " << f.name << "  " << f.version << "  (" << f.path << ")\n";
+        return 0;
+    }
+    auto it = std::find_if(reg.begin(), reg.end(), [&](const Found &f) { return f.name == scene; });
+    if (it == reg.end()) {
+        std::cerr << "Unknown scene: " << scene << "\nAvailable scenes:\n";
+        for (const auto &f : reg) std::cerr << "  " << f.name << "\n";
+        return 1;
+    }
+    pcr::Scene sd;
+    try {
+        sd = pcr::load_scene(it->path);
+    } catch (const std::exception &e) {
+        std::cerr << "Error loading scene: " << e.what() << "\n";
+        return 1;
+    }
+    pcr::HipRenderer r{width, height, depth, samples, shadow};
+    r.useDenoise = denoise;
+    r.useMIS = mis;
+    r.useRussian = russian;
+    r.useStratified = strat;
+    r.useACES = aces;
+    r.aaSamples = aa ? std::max(1, aaSamples) : 1;
+    r.useAdaptive = adaptive;
+    r.useOIDN = oidn;
+    r.useSpectral = spectral;
+    r.heroSamples = hero;
+    r.seed = seed;
+    r.device = device;
+    r.render(sd, start, outputDir);
+    return r.lastOutputPath.empty() ? 1 : 0;
+}

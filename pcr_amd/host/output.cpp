@@ -1,0 +1,127 @@
+// output.cpp -- tone mapping and PNG output of a rendered frame.
+//
+// Tone curves: Reinhard c/(c+1) and Narkowicz's ACES fit (code/Includes/
+// ToneMap.h:14-39).  Quantisation as the reference GPU backends do it: clamp
+// to [0,1], then (unsigned char)(v*255 + 0.5) (OpenglRenderer.cpp:2317-2328).
+// PNG: 8-bit RGB, each row with filter type 0, zlib-compressed IDAT, and tEXt
+// chunks placed before IDAT (code/Includes/PngText.h:32-63).
+#include "output.h"
+
+#include <zlib.h>
+
+#include <cstdio>
+#include <cstring>
+
+namespace pcr {
+
+void tonemap_reinhard(Vec3 &c) {
+    c.x = c.x / (c.x + 1.f);
+    c.y = c.y / (c.y + 1.f);
+    c.z = c.z / (c.z + 1.f);
+}
+
+void tonemap_aces(Vec3 &c) {
+    const float A = 2.51f, B = 0.03f, C = 2.43f, D = 0.59f, E = 0.14f;
+    float *v[3] = {&c.x, &c.y, &c.z};
+    for (float *p : v) {
+        float x = *p;
+        float y = (x * (A * x + B)) / (x * (C * x + D) + E);
+        *p = y < 0.f ? 0.f : (y > 1.f ? 1.f : y);
+    }
+}
+
+uint8_t quantize_clamped(float v) {
+    if (v < 0.f) v = 0.f;
+    if (v > 1.f) v = 1.f;
+    return (uint8_t)(v * 255.f + 0.5f);
+}
+
+std::vector<uint8_t> frame_to_rgb8(const std::vector<Vec3> &frame, bool aces) {
+    std::vector<uint8_t> out(frame.size() * 3);
+    for (size_t i = 0; i < frame.size(); i++) {
+        Vec3 c = frame[i];
+        if (aces) tonemap_aces(c);
+        else tonemap_reinhard(c);
+        out[3 * i] = quantize_clamped(c.x);
+        out[3 * i + 1] = quantize_clamped(c.y);
+        out[3 * i + 2] = quantize_clamped(c.z);
+    }
+    return out;
+}
+
+namespace {
+
+void put_u32(std::vector<uint8_t> &b, uint32_t v) {
+    b.push_back((uint8_t)(v >> 24));
+    b.push_back((uint8_t)(v >> 16));
+    b.push_back((uint8_t)(v >> 8));
+    b.push_back((uint8_t)v);
+}
+
+void chunk(std::vector<uint8_t> &png, const char type[4], const uint8_t *data, size_t n) {
+    put_u32(png, (uint32_t)n);
+    size_t start = png.size();
+    png.insert(png.end(), type, type + 4);
+    if (n) png.insert(png.end(), data, data + n);
+    uLong crc = crc32(0L, Z_NULL, 0);
+    crc = crc32(crc, png.data() + start, (uInt)(png.size() - start));
+    put_u32(png, (uint32_t)crc);
+}
+
+}  // namespace
+
+bool encode_png(const std::vector<uint8_t> &rgb, int w, int h,
+                const std::vector<std::pair<std::string, std::string>> &text, std::vector<uint8_t> &png,
+                std::string *err) {
+    if (w <= 0 || h <= 0 || rgb.size() != (size_t)w * h * 3) {
+        if (err) *err = "bad image size";
+        return false;
+    }
+    png.clear();
+    const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    png.insert(png.end(), sig, sig + 8);
+    std::vector<uint8_t> ihdr;
+    put_u32(ihdr, (uint32_t)w);
+    put_u32(ihdr, (uint32_t)h);
+    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit, truecolour, deflate, adaptive filter, no interlace
+    chunk(png, "IHDR", ihdr.data(), ihdr.size());
+    for (const auto &[k, v] : text) {
+        if (k.empty() || k.size() > 79) {
+            if (err) *err = "tEXt keyword must be 1..79 bytes";
+            return false;
+        }
+        std::vector<uint8_t> body(k.begin(), k.end());
+        body.push_back(0);
+        body.insert(body.end(), v.begin(), v.end());
+        chunk(png, "tEXt", body.data(), body.size());
+    }
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)h * (3 * w + 1));
+    for (int y = 0; y < h; y++) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgb.begin() + (size_t)y * w * 3, rgb.begin() + (size_t)(y + 1) * w * 3);
+    }
+    uLongf zn = compressBound((uLong)raw.size());
+    std::vector<uint8_t> z(zn);
+    if (compress2(z.data(), &zn, raw.data(), (uLong)raw.size(), 6) != Z_OK) {
+        if (err) *err = "zlib compression failed";
+        return false;
+    }
+    chunk(png, "IDAT", z.data(), zn);
+    chunk(png, "IEND", nullptr, 0);
+    return true;
+}
+
+bool write_file(const std::string &path, const std::vector<uint8_t> &data, std::string *err) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        if (err) *err = "cannot open " + path + " for writing";
+        return false;
+    }
+    bool ok = std::fwrite(data.data(), 1, data.size(), f) == data.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok && err) *err = "write failed: " + path;
+    return ok;
+}
+
+}  // namespace pcr

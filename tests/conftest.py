@@ -22,3 +22,11 @@ def golden():
 @pytest.fixture(scope="session")
 def scenes_dir():
     return os.path.join(ROOT, "pcr_amd", "scenes")
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    import pcr_amd
+    ctx = pcr_amd.Context(0)
+    yield ctx
+    ctx.close()

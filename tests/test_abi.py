@@ -1,0 +1,34 @@
+"""The C-ABI library loads and exports every function include/pcrhip.h
+declares (no device calls: CPU only)."""
+import ctypes as C
+import os
+import re
+
+import pcr_amd
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def declared():
+    src = open(os.path.join(HERE, "..", "include", "pcrhip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(pcrhip_[a-z_]+)\s*\(", src)))
+
+
+def test_exports_every_declared_symbol():
+    names = declared()
+    assert len(names) >= 18
+    lib = C.CDLL(pcr_amd.HIP_LIB)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(pcr_amd.HIP_API), set(names) ^ set(pcr_amd.HIP_API)
+
+
+def test_abi_version_and_errors():
+    L = pcr_amd.hip()
+    assert L.pcrhip_abi_version() == 1
+    # argument validation happens before any device call
+    assert L.pcrhip_destroy(None) == pcr_amd.PCRHIP_EINVAL
+    assert b"null" in L.pcrhip_last_error()
+    assert L.pcrhip_scene_release(None) == pcr_amd.PCRHIP_EINVAL
+    assert L.pcrhip_job_wait(None, 0) == pcr_amd.PCRHIP_EINVAL

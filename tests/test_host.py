@@ -1,0 +1,126 @@
+"""Host library (libpcrhost.so): scene ingestion, BVH builder, spectral fits
+vs the compiled reference's fixtures.  CPU only."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import pcr_amd
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = os.path.join(HERE, "golden")
+REF = ["cornell", "cornell-glass", "cornell-spec", "cornell-large-light", "cornell-spheres",
+       "closed-cornell-glass", "closed-cornell-spec", "bright-cornell-glass"]
+OWN = ["cornell-c0", "cornell-c1", "closed-c3", "closed-c4"]
+CASES = [("ref_" + n, os.path.join(G, "ref", "Scenes", n + ".json")) for n in REF] + \
+        [(n, os.path.join(HERE, "..", "pcr_amd", "scenes", n + ".json")) for n in OWN]
+
+
+def u32(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def host_bvh(tris):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    n = len(tris)
+    nodes = np.zeros((max(1, 2 * n), 9), np.int32)
+    perm = np.zeros(max(1, n), np.int32)
+    nn = pcr_amd.host().pcrhost_bvh_build(pcr_amd.fp(tris), n, nodes.ctypes.data_as(pcr_amd._i32),
+                                          perm.ctypes.data_as(pcr_amd._i32))
+    return nodes[:nn], perm[:n]
+
+
+@pytest.mark.parametrize("key,path", CASES, ids=[c[0] for c in CASES])
+def test_ingestion_matches_reference(key, path):
+    g = np.load(os.path.join(G, "scene_%s.npz" % key))
+    s = pcr_amd.HostScene(path)
+    fa, fe = s.fits()
+    assert np.array_equal(u32(fa), u32(g["fit_albedo"]))
+    assert np.array_equal(u32(fe), u32(g["fit_emissive"]))
+    assert np.array_equal(s.nodes(), g["nodes"])
+    d = s.desc
+    if d.n_triangles:
+        tris = np.ctypeslib.as_array(d.triangles, shape=(d.n_triangles * 9,)).reshape(-1, 9)
+        assert np.array_equal(u32(tris), u32(g["leaf_tris"]))
+    areas = np.ctypeslib.as_array(d.light_area, shape=(d.n_lights,))
+    assert np.array_equal(u32(areas), u32(g["light_area"]))
+
+
+def test_bvh_mt19937(golden):
+    g = np.load(golden + "/bvh_mt19937.npz")
+    nodes, perm = host_bvh(g["tris"])
+    assert np.array_equal(nodes, g["nodes"]) and np.array_equal(perm, g["perm"])
+
+
+def test_bvh_c2_digest(golden, scenes_dir):
+    g = np.load(golden + "/bvh_c2.npz")
+    s = pcr_amd.HostScene(os.path.join(scenes_dir, "cornell-c2.json"))
+    assert s.desc.n_triangles == 81932
+    nodes = s.nodes()
+    assert len(nodes) == int(g["n_nodes"])
+    assert sha(nodes) == str(g["nodes_sha256"])
+    assert sha(s.tri_orig()) == str(g["perm_sha256"])
+
+
+def test_degenerate_bvh():
+    nodes, _ = host_bvh(np.zeros((0, 9), np.float32))
+    assert len(nodes) == 0
+    nodes, perm = host_bvh(np.array([[-1, -1, -5, 1, -1, -5, 0, 1, -5]], np.float32))
+    assert len(nodes) == 1 and nodes[0, 8] == 1 and perm[0] == 0
+
+
+def test_fits(golden):
+    g = np.load(golden + "/spectral.npz")
+    out = np.zeros(4, np.float32)
+    for rgb, coef in zip(g["fit_rgb"], g["fit_coef"]):
+        pcr_amd.host().pcrhost_fit_sigmoid(pcr_amd.fp(np.ascontiguousarray(rgb, np.float32)), pcr_amd.fp(out))
+        assert np.array_equal(u32(out), u32(coef)), rgb
+
+
+def test_spd(golden, scenes_dir):
+    g = np.load(golden + "/spectral.npz")
+    out = np.zeros(61, np.float32)
+    for name in ("white", "red", "green", "flat"):
+        rc = pcr_amd.host().pcrhost_load_spd(os.path.join(scenes_dir, "spd", "pcr", name + ".spd").encode(),
+                                             pcr_amd.fp(out))
+        assert rc == 0
+        assert np.array_equal(u32(out), u32(g["spd_pcr_" + name]))
+
+
+def test_loader_errors(tmp_path):
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"schema": "2.0", "name": "x", "version": "1", "materials": {}, "primitives": []}')
+    with pytest.raises(RuntimeError, match="schema major version 2"):
+        pcr_amd.HostScene(str(bad))
+    bad.write_text('{"schema": "1.0", "name": "x", "version": "1",\n  "materials": {"a": {"albedo": [1,1]}}}')
+    with pytest.raises(RuntimeError, match="expected \\[x, y, z\\]"):
+        pcr_amd.HostScene(str(bad))
+    bad.write_text('{"schema": "1.0", "name": "x", "version": "1", "materials": {"a": {"albedo": [1,1,1]}},'
+                   ' "primitives": [{"type": "sphere", "center": [0,0,0], "radius": 1, "material": "a"}]}')
+    with pytest.raises(RuntimeError, match="at least one primitive must have"):
+        pcr_amd.HostScene(str(bad))
+    bad.write_text('{"schema": "1.0", "name": "x", // comment\n "version": "1", oops}')
+    with pytest.raises(RuntimeError, match="bad.json:2:"):
+        pcr_amd.HostScene(str(bad))
+    bad.write_text('{"schema": "1.0", "name": "x", "version": "1", "materials": {"a": {"albedo_spd": "nope"}},'
+                   ' "primitives": []}')
+    with pytest.raises(RuntimeError, match="SPD 'nope' not found"):
+        pcr_amd.HostScene(str(bad))
+
+
+def test_png_roundtrip(tmp_path):
+    from PIL import Image
+    rgb = (np.arange(4 * 3 * 3) * 7 % 256).astype(np.uint8).reshape(4, 3, 3)
+    p = str(tmp_path / "x.png")
+    kv = b"Scene\0cornell\0Depth\x004\0\0"
+    assert pcr_amd.host().pcrhost_write_png(p.encode(), rgb.ctypes.data_as(pcr_amd._u8), 3, 4, kv) == 0
+    im = Image.open(p)
+    assert np.array_equal(np.asarray(im.convert("RGB")), rgb)
+    assert im.text == {"Scene": "cornell", "Depth": "4"}
+    raw = open(p, "rb").read()
+    assert raw.index(b"tEXt") < raw.index(b"IDAT")
