@@ -41,25 +41,26 @@ def parse():
 
 def cpu_baseline(scene_path, target_s):
     """The CPU oracle's non-branching estimator on a bounded sample of the same
-    frame (full-width row bands, 1 sample per pixel), all host threads."""
+    frame: the full 720x720 frame at the first k samples per pixel, k chosen
+    from a 1-sample calibration run so the sample takes about target_s; all
+    host threads (at most 16, the box's CPU share)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import scene_io
     osc = scene_io.build_oracle_scene(scene_io.parse_scene(scene_path))
     threads = max(1, min(16, os.cpu_count() or 1))
-    W, H, d, S = WORKLOAD["width"], WORKLOAD["height"], WORKLOAD["depth"], WORKLOAD["shadow"]
-    rows = 8
-    while True:
+    W, H, d, S, spp = WORKLOAD["width"], WORKLOAD["height"], WORKLOAD["depth"], WORKLOAD["shadow"], WORKLOAD["spp"]
+
+    def run(k):
         t0 = time.perf_counter()
-        _, cnt = osc.render_paths(W, H, d, WORKLOAD["spp"], S, 7, crop=(0, H // 2 - rows // 2, W, rows),
-                                  s_range=(0, 1), threads=threads)
-        dt = time.perf_counter() - t0
-        if dt >= target_s * 0.5 or rows >= H:
-            break
-        rows = min(H, max(rows + 1, int(rows * target_s / max(dt, 1e-3))))
-    paths = int(cnt[0])
+        _, cnt = osc.render_paths(W, H, d, spp, S, 7, s_range=(0, k), threads=threads)
+        return time.perf_counter() - t0, int(cnt[0])
+
+    t1, _ = run(1)
+    k = max(1, min(spp, int(round(target_s / max(t1, 1e-3)))))
+    dt, paths = run(k)
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": "oracle non-branching estimator, %dx%d rows of the 720x720 frame, 1 spp, depth %d, "
-                      "%d shadow rays, spectral, %d paths in %.1f s" % (W, rows, d, S, paths, dt)}
+            "sample": "oracle non-branching estimator: the 720x720 frame at samples 0..%d of 64, depth %d, "
+                      "%d shadow rays, spectral; %d paths in %.1f s on %d threads" % (k - 1, d, S, paths, dt, threads)}
 
 
 def main():

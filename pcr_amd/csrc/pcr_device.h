@@ -224,9 +224,16 @@ __device__ __forceinline__ bool tri_test(const float4 *__restrict__ tri, int ti,
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
     if (fabsf(det) < 1e-6f) return false;
-    float inv = div_rn(1.f, det);
     V3 tv = sub(o, v0);
-    float u = fmul(dot(tv, pv), inv);
+    float du = dot(tv, pv);
+    // u = du * (1/det) with both roundings correct.  Decided without the
+    // division where the outcome is certain: u < 0 exactly when du and det
+    // have strictly opposite signs (du = +-0 gives u = +-0), and
+    // |du| > |det| * 1.000001 forces u > 1 + 2^-23 after both roundings.
+    if ((du < 0.f && det > 0.f) || (du > 0.f && det < 0.f)) return false;
+    if (fabsf(du) > fmul(fabsf(det), 1.000001f)) return false;
+    float inv = div_rn(1.f, det);
+    float u = fmul(du, inv);
     if (u < 0.f || u > 1.f) return false;
     V3 qv = cross(tv, e1);
     float v = fmul(dot(d, qv), inv);
@@ -240,25 +247,28 @@ __device__ __forceinline__ bool tri_test(const float4 *__restrict__ tri, int ti,
     return true;
 }
 // Slab test against [lo, hi] for the segment (0, seg); inv = 1/d per axis.
+// tmin and tmax are never NaN, and fmaxf(a, NaN) = a = std::max(a, NaN),
+// fminf(a, NaN) = a = std::min(a, NaN); for other operands they agree up to
+// the sign of a zero, which no comparison distinguishes.
 __device__ __forceinline__ bool box_test(V3 o, V3 inv, float lx, float ly, float lz, float hx, float hy, float hz,
                                          float seg, float &tnear) {
     float tmin = 0.f, tmax = seg;
     float t1 = fmul(fsub(lx, o.x), inv.x), t2 = fmul(fsub(hx, o.x), inv.x);
     if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
-    tmin = smax(tmin, t1);
-    tmax = smin(tmax, t2);
+    tmin = fmaxf(tmin, t1);
+    tmax = fminf(tmax, t2);
     if (tmin > tmax) return false;
     t1 = fmul(fsub(ly, o.y), inv.y);
     t2 = fmul(fsub(hy, o.y), inv.y);
     if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
-    tmin = smax(tmin, t1);
-    tmax = smin(tmax, t2);
+    tmin = fmaxf(tmin, t1);
+    tmax = fminf(tmax, t2);
     if (tmin > tmax) return false;
     t1 = fmul(fsub(lz, o.z), inv.z);
     t2 = fmul(fsub(hz, o.z), inv.z);
     if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
-    tmin = smax(tmin, t1);
-    tmax = smin(tmax, t2);
+    tmin = fmaxf(tmin, t1);
+    tmax = fminf(tmax, t2);
     if (tmin > tmax) return false;
     tnear = tmin;
     return true;

@@ -661,7 +661,7 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
     c->n_cu = prop.multiProcessorCount;
     c->cfg.block = 128;
     c->cfg.blocks_per_cu = 8;
-    c->cfg.target_pass_paths = 4u << 20;
+    c->cfg.target_pass_paths = 16u << 20;
     c->cfg.memory_budget = 64ull << 30;
     c->cfg.max_active_jobs = 32;
     auto cleanup = [&](int rc) {

@@ -68,7 +68,9 @@ __device__ __forceinline__ void st_store(const QueueView &q, uint32_t i, const P
 // wavelength of hero channel c (Renderer.cpp:227-239)
 __device__ __forceinline__ float hero_lambda(float l0, int c, int N) {
     if (c >= N || c == 0) return l0;
-    float stride = div_rn(300.f, (float)N);
+    // 300/N for N = 1..4 (exactly representable except N = 3, whose correctly
+    // rounded quotient is 100)
+    float stride = N == 4 ? 75.f : (N == 2 ? 150.f : (N == 3 ? 100.f : 300.f));
     float l = fadd(l0, fmul(stride, (float)c));
     if (l > 700.f) l = fsub(l, 300.f);
     return l;
@@ -184,7 +186,7 @@ __device__ __forceinline__ void write_radiance(const DJob &J, const PathState &p
     if (ch0 == 0 && nch >= 3) {
         J.radiance[ps.path] = make_float4(ps.rad[0], ps.rad[1], ps.rad[2], ps.rad[3]);
     } else {
-        for (int c = 0; c < nch; c++) r[ch0 + c] = ps.rad[c];
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) r[ch0 + c] = ps.rad[c];
     }
 }
 
@@ -192,13 +194,22 @@ struct BounceCounters {
     uint32_t ext, shadow, emit;
 };
 
-// One bounce of one path.  Returns the number of output rays written to out[]
-// (0: terminated, 1: continues, up to 4: dispersion fork).
+// Appends the sub-paths of a dispersion fork straight to the output queue with
+// one atomic for the lane (forks are rare: one glass hit per path at most).
+struct ForkSink {
+    const QueueView *out;
+    uint32_t *count;
+    uint32_t capacity;
+    uint32_t *overflow;
+};
+
+// One bounce of one path.  Updates `ps` in place and returns true if it
+// continues as a single ray; terminated paths (and forks, whose sub-paths go
+// to the sink) return false.
 template <int STACK>
-__device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathState &ps, PathState *out,
-                                           const LdsStack &st, BounceCounters &cnt) {
+__device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const LdsStack &st, BounceCounters &cnt,
+                                            const ForkSink &fork) {
     const int slot = st_slot(ps.meta);
-    const DJob &J = jobs[slot];
     const DScene &S = J.S;
     const bool spectral = (J.flags & JF_SPECTRAL) != 0;
     const int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
@@ -206,10 +217,10 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
     const bool first = st_first(ps.meta);
     if (bounce >= J.depth) {
         write_radiance(J, ps);
-        return 0;
+        return false;
     }
 
-    float lam[4];
+    float lam[4] = {0.f, 0.f, 0.f, 0.f};
     if (spectral) {
         if (nch == 1 && ch0 > 0) lam[0] = hero_lambda(ps.lam0, ch0, J.hero);
         else
@@ -220,25 +231,25 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
     HitRec h = scene_closest<STACK>(S, ps.o, ps.d, 3.40282347e38f, st);
     if (h.kind < 0) {
         write_radiance(J, ps);
-        return 0;
+        return false;
     }
     V3 hit, N;
     int mi;
     hit_geometry(S, h, ps.o, ps.d, hit, N, mi);
     bool entering = dot(ps.d, N) < 0.f;
     if (!entering) N = mul(N, -1.f);
-    const DMat M = S.mat[mi];
+    const DMat &M = S.mat[mi];
     if (M.flags & MF_EMISSIVE) {
         cnt.emit++;
-        for (int c = 0; c < nch; c++) {
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) {
             float le = spectral ? emissive_at(M, S.tabs, lam[c]) : (c == 0 ? M.emissive.x : (c == 1 ? M.emissive.y : M.emissive.z));
             ps.rad[c] = fadd(ps.rad[c], fmul(ps.tp[c], le));
         }
         write_radiance(J, ps);
-        return 0;
+        return false;
     }
-    float alb[4];
-    for (int c = 0; c < nch; c++)
+    float alb[4] = {0.f, 0.f, 0.f, 0.f};
+    _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch)
         alb[c] = spectral ? albedo_at(M, S.tabs, lam[c]) : (c == 0 ? M.albedo.x : (c == 1 ? M.albedo.y : M.albedo.z));
 
     const int nb = bounce + 1;
@@ -246,22 +257,22 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
         float cosI = -dot(ps.d, N);
         ps.d = add(ps.d, mul(N, fmul(2.f, cosI)));
         ps.o = add(hit, mul(N, kEps3));
-        for (int c = 0; c < nch; c++) ps.tp[c] = fmul(ps.tp[c], alb[c]);
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.tp[c] = fmul(ps.tp[c], alb[c]);
         ps.meta = st_pack(nb, false, nch, ch0, slot);
         if (nb >= J.depth) {
             write_radiance(J, ps);
-            return 0;
+            return false;
         }
-        out[0] = ps;
-        return 1;
+        return true;
     }
     if (M.flags & MF_TRANSPARENT) {
         const float ior = M.albedo.w, cB = M.emissive.w;
         if (spectral && cB != 0.f && nch > 1) {
             // Dispersion fork: one single-wavelength sub-path per hero channel,
             // each on its own stream derived from the parent's state.
-            int n = 0;
-            for (int c = 0; c < nch; c++) {
+            uint32_t base = 0;
+            if (nb < J.depth) base = atomicAdd(fork.count, (uint32_t)nch);
+            _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) {
                 PathState ch = ps;
                 ch.rng = stream_seed(ps.rng, (uint64_t)(c + 1));
                 float iorc = cauchy_ior(ior, cB, lam[c]);
@@ -275,28 +286,35 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
                 ch.meta = st_pack(nb, false, 1, ch0 + c, slot);
                 if (nb >= J.depth) {
                     write_radiance(J, ch);
+                } else if (base + (uint32_t)c < fork.capacity) {
+                    st_store(*fork.out, base + (uint32_t)c, ch);
                 } else {
-                    out[n++] = ch;
+                    atomicOr(fork.overflow, 1u);
                 }
             }
-            return n;
+            return false;
         }
         float iorx = (spectral && nch == 1) ? cauchy_ior(ior, cB, lam[0]) : ior;
         Bounce b = dielectric(ps.d, N, hit, entering, iorx, rng_next(ps.rng));
         ps.d = b.dir;
         ps.o = b.origin;
-        for (int c = 0; c < nch; c++) ps.tp[c] = fmul(ps.tp[c], alb[c]);
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.tp[c] = fmul(ps.tp[c], alb[c]);
         ps.meta = st_pack(nb, false, nch, ch0, slot);
         if (nb >= J.depth) {
             write_radiance(J, ps);
-            return 0;
+            return false;
         }
-        out[0] = ps;
-        return 1;
+        return true;
     }
     // diffuse: next-event estimation (Renderer.cpp:565-653, OpenglRenderer.cpp:1150-1227)
     if (S.total_area > 0.f) {
         float dl[4] = {0.f, 0.f, 0.f, 0.f};
+        // loop invariants of the reference's per-sample expression
+        // (albedo/pi)*Le*G*A*w: albedo/pi per channel, and Le per channel for
+        // the last light material seen
+        float aop[4], le_c[4] = {0.f, 0.f, 0.f, 0.f};
+        _Pragma("unroll") for (int c = 0; c < 4; c++) aop[c] = div_rn(alb[c], kPiF);
+        int le_mat = -1;
         for (int i = 0; i < J.shadow; i++) {
             int li = 0;
             if (S.n_lights > 1) {
@@ -307,7 +325,7 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
                     if (target <= cum) { li = k; break; }
                 }
             }
-            const DLight L = S.lights[li];
+            const DLight &L = S.lights[li];
             V3 sP, sN;
             int lmat;
             if (L.kind == 0) {
@@ -341,7 +359,13 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
             float d2 = dot(Li, Li);
             V3 so = cosT <= 0 ? sub(hit, mul(N, kEps3)) : add(hit, mul(N, kEps3));
             cnt.shadow++;
-            HitRec sh = scene_closest<STACK>(S, so, wi, 3.40282347e38f, st);
+            // Bounded segment: a hit at t >= sqrt(d2)*1.001 + 1 gives
+            // |hit - so|^2 > d2 - 1e-3, i.e. never occludes, and bounding the
+            // closest-hit search there leaves every hit below it -- and so the
+            // occlusion answer of the reference's unbounded search -- unchanged.
+            float tb = fadd(fmul(sqrt_rn(d2), 1.001f), 1.f);
+            if (!(tb < 3.40282347e38f)) tb = 3.40282347e38f;
+            HitRec sh = scene_closest<STACK>(S, so, wi, tb, st);
             bool occl = false;
             if (sh.kind >= 0) {
                 V3 sp, sn;
@@ -359,24 +383,27 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
                     float pb = div_rn(cosT, kPiF);
                     w = div_rn(fmul(pl, pl), fadd(fmul(pl, pl), fmul(pb, pb)));
                 }
-                const DMat LM = S.mat[lmat];
-                for (int c = 0; c < nch; c++) {
-                    float le = spectral ? emissive_at(LM, S.tabs, lam[c])
-                                        : (c == 0 ? LM.emissive.x : (c == 1 ? LM.emissive.y : LM.emissive.z));
-                    dl[c] = fadd(dl[c], fmul(fmul(fmul(fmul(div_rn(alb[c], kPiF), le), G), S.total_area), w));
+                if (lmat != le_mat) {
+                    const DMat &LM = S.mat[lmat];
+                    _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch)
+                        le_c[c] = spectral ? emissive_at(LM, S.tabs, lam[c])
+                                           : (c == 0 ? LM.emissive.x : (c == 1 ? LM.emissive.y : LM.emissive.z));
+                    le_mat = lmat;
                 }
+                _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch)
+                    dl[c] = fadd(dl[c], fmul(fmul(fmul(fmul(aop[c], le_c[c]), G), S.total_area), w));
             }
         }
-        for (int c = 0; c < nch; c++) ps.rad[c] = fadd(ps.rad[c], fmul(ps.tp[c], div_rn(dl[c], (float)J.shadow)));
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.rad[c] = fadd(ps.rad[c], fmul(ps.tp[c], div_rn(dl[c], (float)J.shadow)));
     }
     if ((J.flags & JF_RUSSIAN) && bounce >= 1) {
         float a = spectral ? alb[0] : smax(smax(alb[0], alb[1]), alb[2]);
         float p = smin(0.95f, smax(0.05f, a));
         if (rng_next(ps.rng) > p) {
             write_radiance(J, ps);
-            return 0;
+            return false;
         }
-        for (int c = 0; c < nch; c++) ps.tp[c] = div_rn(ps.tp[c], p);
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.tp[c] = div_rn(ps.tp[c], p);
     }
     float r1, r2;
     if ((J.flags & JF_STRATIFIED) && first) {
@@ -387,14 +414,13 @@ __device__ __forceinline__ int bounce_path(const DJob *__restrict__ jobs, PathSt
     }
     ps.d = fm_hemisphere(N, r1, r2);
     ps.o = add(hit, mul(N, kEps3));
-    for (int c = 0; c < nch; c++) ps.tp[c] = fmul(ps.tp[c], alb[c]);
+    _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.tp[c] = fmul(ps.tp[c], alb[c]);
     ps.meta = st_pack(nb, false, nch, ch0, slot);
     if (nb >= J.depth) {
         write_radiance(J, ps);
-        return 0;
+        return false;
     }
-    out[0] = ps;
-    return 1;
+    return true;
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -436,6 +462,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
     const uint32_t n_in = FIRST ? a.n_gen : min(*a.count_in, a.capacity);
     LaneCounters lc;
     const uint32_t lane = lane_id();
+    const ForkSink fork{&a.out, a.count_out, a.capacity, a.overflow};
     for (;;) {
         if (threadIdx.x == 0) s_base = atomicAdd(a.work, (uint32_t)BLOCK);
         __syncthreads();
@@ -443,28 +470,39 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
         __syncthreads();
         if (base >= n_in) break;
         const uint32_t i = base + threadIdx.x;
-        PathState outs[4];
-        int nout = 0;
+        PathState ps;
+        bool ok = false;
+        int slot = -1;
+        uint32_t gpath = 0;
         if (i < n_in) {
-            PathState ps;
-            bool ok = true;
-            int slot = 0;
             if (FIRST) {
-                uint32_t path;
-                ok = gen_lookup(a.segs, a.n_segs, i, slot, path);
-                if (ok) gen_path(a.jobs[slot], slot, path, ps);
+                ok = gen_lookup(a.segs, a.n_segs, i, slot, gpath);
             } else {
                 st_load(a.in, i, ps);
                 slot = st_slot(ps.meta);
+                ok = true;
             }
-            if (ok) {
-                if (slot != lc.slot) {
+        }
+        // Waterfall over the job slots present in the wave: each round takes
+        // the first pending lane's slot as a wave-uniform value, so the job
+        // descriptor and scene are read with scalar loads.  A wave of a single
+        // job (the common case) runs one round.
+        bool alive = false;
+        bool pending = ok;
+        while (__ballot(pending)) {
+            const uint64_t pm = __ballot(pending);
+            const int s0 = __builtin_amdgcn_readfirstlane(__shfl(slot, __ffsll((unsigned long long)pm) - 1, 64));
+            if (pending && slot == s0) {
+                pending = false;
+                if (s0 != lc.slot) {
                     if (lc.slot >= 0) flush_lane(lc, a.counters);
                     lc = LaneCounters{};
-                    lc.slot = slot;
+                    lc.slot = s0;
                 }
+                const DJob &J = a.jobs[s0];
+                if (FIRST) gen_path(J, s0, gpath, ps);
                 BounceCounters cnt{0u, 0u, 0u};
-                nout = bounce_path<STACK>(a.jobs, ps, outs, st, cnt);
+                alive = bounce_path<STACK>(J, ps, st, cnt, fork);
                 lc.paths += FIRST ? 1u : 0u;
                 lc.ext += cnt.ext;
                 lc.shadow += cnt.shadow;
@@ -472,29 +510,17 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
             }
         }
         // compaction: a wave reserves its output range with one atomic
-        const uint64_t multi = __ballot(nout > 1);
-        uint32_t off, total;
-        if (multi == 0) {
-            const uint64_t m = __ballot(nout == 1);
-            off = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            total = (uint32_t)__popcll(m);
-        } else {
-            uint32_t x = (uint32_t)nout;
-            for (int dd = 1; dd < 64; dd <<= 1) {
-                uint32_t y = __shfl_up(x, dd, 64);
-                if (lane >= (uint32_t)dd) x += y;
-            }
-            total = __shfl(x, 63, 64);
-            off = x - (uint32_t)nout;
-        }
+        const uint64_t m = __ballot(alive);
+        const uint32_t off = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t total = (uint32_t)__popcll(m);
         uint32_t wbase = 0;
         if (total) {
             if (lane == 0) wbase = atomicAdd(a.count_out, total);
             wbase = __shfl(wbase, 0, 64);
         }
-        for (int k = 0; k < nout; k++) {
-            uint32_t dst = wbase + off + (uint32_t)k;
-            if (dst < a.capacity) st_store(a.out, dst, outs[k]);
+        if (alive) {
+            uint32_t dst = wbase + off;
+            if (dst < a.capacity) st_store(a.out, dst, ps);
             else atomicOr(a.overflow, 1u);
         }
     }
@@ -565,7 +591,7 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
             float l0 = fadd(400.f, fmul(rng_next(rng), 300.f));
             float rr[4] = {r.x, r.y, r.z, r.w};
             float X = 0.f, Y = 0.f, Z = 0.f;
-            for (int c = 0; c < N; c++) {
+            _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < N) {
                 V3 cmf = fm_cmf(hero_lambda(l0, c, N));
                 X = fadd(X, fmul(fmul(rr[c], cmf.x), sc));
                 Y = fadd(Y, fmul(fmul(rr[c], cmf.y), sc));
