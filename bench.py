@@ -24,6 +24,10 @@ WORKLOAD = dict(scene="cornell-c1", width=720, height=720, depth=8, spp=64, shad
 # indices 16), per shadow ray 120 B, per emissive hit 24 B.
 BYTES_PATH, BYTES_EXT, BYTES_SHADOW, BYTES_EMIT = 64, 176, 120, 24
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_bounce launch at the default configuration, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (factors 2.0 and 1.0):
+# profiles/k_bounce_cornell-c1_720x720_d8_64spp_hbm_traffic.md.
+TRAFFIC_BYTES_PER_LAUNCH = 2147981885
 
 
 def parse():
@@ -36,7 +40,41 @@ def parse():
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--pass-paths", type=int, default=0)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run 3 device-to-device copies of MIB MiB each (the reference workload for "
+                        "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_copy": 4 * n, "copies": 3}), flush=True)
+
+
+def copy_gbs(mib=1024, reps=5):
+    """Attainable HBM bandwidth: (read + write) bytes of a device-to-device
+    copy over its event-timed duration."""
+    import torch
+    a = torch.ones(mib * (1 << 18), device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    gbs = 2.0 * a.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def cpu_baseline(scene_path, target_s):
@@ -69,6 +107,9 @@ def main():
         print("bench.py measures one GPU: every box in this pool has a single MI355X; "
               "multi-GPU runs are not supported by this benchmark.", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     import pcr_amd
     scene_path = os.path.join(ROOT, "pcr_amd", "scenes", WORKLOAD["scene"] + ".json")
     ctx = pcr_amd.Context(0)
@@ -132,7 +173,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if not cfg else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per k_bounce dispatch",
+            "copy_gbs": round(copy_gbs(), 1),
             "launches": prof.bounce_launches,
             "avg_launch_ms": round(prof.bounce_ms / max(1, prof.bounce_launches), 4),
             "algorithmic_bytes_per_launch": alg_bytes // max(1, prof.bounce_launches),
