@@ -24,10 +24,14 @@ WORKLOAD = dict(scene="cornell-c1", width=720, height=720, depth=8, spp=64, shad
 # indices 16), per shadow ray 120 B, per emissive hit 24 B.
 BYTES_PATH, BYTES_EXT, BYTES_SHADOW, BYTES_EMIT = 64, 176, 120, 24
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_bounce launch at the default configuration, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (factors 2.0 and 1.0):
+# HBM bytes per k_bounce launch, from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes scaled by a 1 GiB copy (factors 2.0 and 1.0), for the pass size they
+# were measured with (the queue records a launch reads and writes depend on the
+# paths per pass, not on the block shape):
 # profiles/k_bounce_cornell-c1_720x720_d8_64spp_hbm_traffic.md.
 TRAFFIC_BYTES_PER_LAUNCH = 2147981885
+TRAFFIC_CONFIG = dict(target_pass_paths=16 << 20)
+SCHEDULES = {0: "graph", 1: "persistent", 2: "staged"}
 
 
 def parse():
@@ -40,6 +44,7 @@ def parse():
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--pass-paths", type=int, default=0)
+    p.add_argument("--no-autotune", action="store_true", help="keep the library's default launch configuration")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run 3 device-to-device copies of MIB MiB each (the reference workload for "
                         "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -120,11 +125,15 @@ def main():
         cfg["blocks_per_cu"] = a.blocks_per_cu
     if a.pass_paths:
         cfg["target_pass_paths"] = a.pass_paths
-    if cfg:
-        ctx.set_config(**cfg)
     ds = ctx.upload(pcr_amd.HostScene(scene_path))
     kw = dict(width=WORKLOAD["width"], height=WORKLOAD["height"], depth=WORKLOAD["depth"], spp=WORKLOAD["spp"],
               shadow=WORKLOAD["shadow"], hero=WORKLOAD["hero"], spectral=True)
+    if not a.no_autotune and not cfg:
+        # untimed: the library's autotuner picks block shape, grid, schedule and pass size
+        ds.autotune(trial_paths=16 << 20, seed=999, **kw)
+    if cfg:
+        ctx.set_config(**cfg)
+    tuned = ctx.config()
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -165,6 +174,11 @@ def main():
         "config": {"workload": "cornell-c1 720x720 depth 8, 64 spp, 4 shadow rays, 4 hero wavelengths",
                    "paths_per_step": paths // a.steps, "ext_rays_per_step": ext // a.steps,
                    "shadow_rays_per_step": shadow // a.steps, "passes_per_step": stats[0].passes,
+                   "launch": {"block": tuned.block, "blocks_per_cu": tuned.blocks_per_cu,
+                              "target_pass_paths": tuned.target_pass_paths,
+                              "schedule": SCHEDULES.get(tuned.schedule),
+                              "autotuned": not a.no_autotune and not cfg,
+                              "note": "profiled passes issue the graph schedule's kernels as direct launches"},
                    "parallelism": "1 GPU"},
         "roofline": {
             "bound": "hbm",
@@ -173,7 +187,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH if not cfg else None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if all(getattr(tuned, k) == v for k, v in TRAFFIC_CONFIG.items())
+            else None,
             "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per k_bounce dispatch",
             "copy_gbs": round(copy_gbs(), 1),
             "launches": prof.bounce_launches,

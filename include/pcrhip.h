@@ -71,12 +71,25 @@ int pcrhip_abi_version(void);
 int pcrhip_create(int device, pcrhip_ctx **out);
 int pcrhip_destroy(pcrhip_ctx *ctx);
 
+/* Pass schedules (pcrhip_config.schedule).  A pass is: reset counters, the
+ * bounce kernel once per bounce, resolve.
+ *   GRAPH      persistent bounce kernels, the whole pass captured once in a
+ *              hipGraph and replayed (sizes live in device memory);
+ *   PERSISTENT the same launches issued one by one;
+ *   STAGED     each bounce's grid sized to its live rays, which the host reads
+ *              back between bounces (the reference Metal wavefront's schedule,
+ *              MetalRenderer.mm:1794-1912). */
+enum { PCRHIP_SCHED_GRAPH = 0, PCRHIP_SCHED_PERSISTENT = 1, PCRHIP_SCHED_STAGED = 2 };
+
+/* Launch and batching knobs; the counterpart of the reference's threadgroup
+ * and samples-per-pass defaults (GpuDefaults.h:15-50, MetalRenderer.mm:3799-3823). */
 typedef struct {
     int block;                   /* threads per block of the bounce kernel: 64, 128 or 256 */
     int blocks_per_cu;           /* persistent grid = CUs x blocks_per_cu */
     uint64_t target_pass_paths;  /* paths per pass (a job's samples are split into passes) */
     uint64_t memory_budget;      /* bytes of per-job buffers the admission may hold */
     int max_active_jobs;         /* concurrent job slots (<= 64) */
+    int schedule;                /* PCRHIP_SCHED_* */
 } pcrhip_config;
 int pcrhip_get_config(pcrhip_ctx *ctx, pcrhip_config *out);
 /* Takes effect for passes started after the call. */
@@ -195,6 +208,25 @@ int pcrhip_job_release(pcrhip_job *job);
 /* Submit + wait + result in one call. */
 int pcrhip_render(pcrhip_ctx *ctx, pcrhip_scene *scene, const pcrhip_job_desc *desc, float *out, size_t n_floats,
                   pcrhip_stats *stats);
+
+/* ---------------------------------------------------------- autotuner ----- */
+/* One autotuner trial: the configuration and its measured speed. */
+typedef struct {
+    int block, blocks_per_cu, schedule;
+    uint64_t target_pass_paths;
+    double ms;        /* best of the trial's repetitions */
+    double mpaths_s;  /* paths / ms */
+} pcrhip_tune_entry;
+/* Autotuner: renders a trial of `desc` (its samples cut to about trial_paths
+ * paths) under candidate configurations -- block x blocks_per_cu, then
+ * schedule, then pass size, each stage keeping the best of the previous --
+ * and leaves the fastest configuration set on the context.  Needs an idle
+ * context (no queued or running jobs).  Writes up to max_entries trials to
+ * table (may be NULL) and their number to n_entries.  The reference picks its
+ * threadgroup shape and pass budget by hand-run A/B tests instead
+ * (GpuDefaults.h:15-50). */
+int pcrhip_autotune(pcrhip_ctx *ctx, pcrhip_scene *scene, const pcrhip_job_desc *desc, uint64_t trial_paths,
+                    pcrhip_tune_entry *table, int max_entries, int *n_entries);
 
 /* ------------------------------------------------------- measurement ----- */
 /* When enabled, the serving loop brackets every bounce-kernel launch with HIP

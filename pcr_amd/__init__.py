@@ -20,6 +20,7 @@ HOST_LIB = os.path.join(BUILD, "libpcrhost.so")
 PCRHIP_OK, PCRHIP_EINVAL, PCRHIP_EDEVICE, PCRHIP_ENOMEM = 0, -1, -2, -3
 PCRHIP_ECANCELLED, PCRHIP_ETIMEOUT, PCRHIP_ESTATE = -4, -5, -6
 SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER = 1, 2, 4, 8, 16
+SCHED_GRAPH, SCHED_PERSISTENT, SCHED_STAGED = 0, 1, 2
 
 _f = C.POINTER(C.c_float)
 _d = C.POINTER(C.c_double)
@@ -46,7 +47,12 @@ class SceneDesc(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("target_pass_paths", C.c_uint64),
-                ("memory_budget", C.c_uint64), ("max_active_jobs", C.c_int)]
+                ("memory_budget", C.c_uint64), ("max_active_jobs", C.c_int), ("schedule", C.c_int)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("schedule", C.c_int),
+                ("target_pass_paths", C.c_uint64), ("ms", C.c_double), ("mpaths_s", C.c_double)]
 
 
 class SceneInfo(C.Structure):
@@ -91,6 +97,8 @@ HIP_API = {
     "pcrhip_job_result_sums": (C.c_int, [C.c_void_p, _d, C.c_size_t]),
     "pcrhip_job_release": (C.c_int, [C.c_void_p]),
     "pcrhip_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), _f, C.c_size_t, C.POINTER(Stats)]),
+    "pcrhip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), C.c_uint64, C.POINTER(TuneEntry),
+                                  C.c_int, C.POINTER(C.c_int)]),
     "pcrhip_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "pcrhip_profile_get": (C.c_int, [C.c_void_p, C.POINTER(Profile), C.c_int]),
 }
@@ -237,6 +245,17 @@ class DeviceScene:
         if self.h:
             check(hip().pcrhip_scene_release(self.h))
             self.h = None
+
+    def autotune(self, trial_paths=16 << 20, **kw):
+        """Runs the library's autotuner on a trial of the job `kw` describes
+        (job_desc arguments) and leaves the fastest configuration on the
+        context.  Returns the trials as dicts, in the order they ran."""
+        table = (TuneEntry * 64)()
+        n = C.c_int()
+        check(hip().pcrhip_autotune(self.ctx.h, self.h, C.byref(job_desc(**kw)), trial_paths, table, 64,
+                                    C.byref(n)))
+        keys = [f for f, _ in TuneEntry._fields_]
+        return [{k: getattr(table[i], k) for k in keys} for i in range(min(n.value, 64))]
 
     def info(self):
         i = SceneInfo()

@@ -50,11 +50,20 @@ struct ResolveSeg {
 
 constexpr int kMaxSegs = 64;
 
+// Sizes of the current pass, read by the kernels from device memory so that
+// the launch arguments stay the same from pass to pass (a captured pass
+// graph is replayed unchanged).
+struct PassInfo {
+    uint32_t n_gen;    // generating bounce: number of work items (paths)
+    int32_t n_segs;    // GenSeg entries
+    uint32_t n_pix;    // resolve: number of pixels
+    int32_t n_rsegs;   // ResolveSeg entries
+};
+
 struct BounceArgs {
     const DJob *jobs;
     const GenSeg *segs;
-    int32_t n_segs;
-    uint32_t n_gen;               // generating bounce: number of work items
+    const PassInfo *pass;
     QueueView in, out;
     const uint32_t *count_in;     // live rays in `in`
     uint32_t *count_out;          // rays appended to `out`
@@ -67,7 +76,7 @@ struct BounceArgs {
 struct ResolveArgs {
     const DJob *jobs;
     const ResolveSeg *segs;
-    int32_t n_segs;
+    const PassInfo *pass;
 };
 
 struct IntersectArgs {
@@ -85,8 +94,9 @@ struct IntersectArgs {
 
 extern "C" {
 hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block, hipStream_t s);
-hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, uint32_t n_items, hipStream_t s);
+// Threads per block the bounce launcher actually uses for a stack depth.
+int pcr_bounce_block(int stack, int block);
+hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, int grid, hipStream_t s);
 hipError_t pcr_launch_intersect(const pcr::IntersectArgs *a, int stack, hipStream_t s);
 hipError_t pcr_launch_reset(uint32_t *p, int n, hipStream_t s);
-hipError_t pcr_launch_advance(pcr::DJob *jobs, const int32_t *slots, int n, hipStream_t s);
 }

@@ -121,9 +121,17 @@ struct pcrhip_ctx {
     ResolveSeg *rsegs_dev = nullptr;
     uint32_t *ctrs = nullptr;  // [2 * kMaxDepthBounces + 2]: per bounce (count_out, work), overflow
     unsigned long long *slot_counters = nullptr;  // [kSlots * 4]
+    PassInfo *pass_dev = nullptr;
     QueueView qa{}, qb{};
     uint64_t qcap = 0;
     DJob *jobs_host = nullptr;  // pinned staging of the job table
+    PassInfo *pass_host = nullptr;  // pinned staging of the pass sizes
+    // captured pass graphs, keyed by launch shape; dropped when the queues move
+    struct PassGraph {
+        int nb, stack, block, grid;
+        hipGraphExec_t exec;
+    };
+    std::vector<PassGraph> graphs;
     // serving
     std::mutex m;
     std::condition_variable cv;       // worker wake-up
@@ -159,8 +167,14 @@ hipError_t alloc_queue(QueueView &q, uint64_t n) {
     return hipMalloc(&q.rng, n * sizeof(uint4));
 }
 
+void drop_graphs(pcrhip_ctx *c) {
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
 int ensure_queues(pcrhip_ctx *c, uint64_t need) {
     if (need <= c->qcap) return PCRHIP_OK;
+    drop_graphs(c);  // captured launches hold the old queue addresses
     free_queue(c->qa);
     free_queue(c->qb);
     c->qcap = 0;
@@ -276,14 +290,98 @@ void admit(pcrhip_ctx *c) {
     }
 }
 
+// Launch shape of one pass.
+struct PassShape {
+    int nb, stack, block, grid, rgrid;
+};
+
+BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
+    BounceArgs a{};
+    a.jobs = c->jobs_dev;
+    a.segs = c->segs_dev;
+    a.pass = c->pass_dev;
+    a.in = (b & 1) ? c->qa : c->qb;
+    a.out = (b & 1) ? c->qb : c->qa;
+    a.count_in = b == 0 ? nullptr : c->ctrs + 2 * (b - 1);
+    a.count_out = c->ctrs + 2 * b;
+    a.work = c->ctrs + 2 * b + 1;
+    a.capacity = (uint32_t)std::min<uint64_t>(c->qcap, 0xFFFFFFFFu);
+    a.overflow = c->ctrs + 2 * sh.nb;
+    a.counters = c->slot_counters;
+    return a;
+}
+
+// Issues a pass on the context stream: counter resets (kernels, so the pass
+// can be captured), the bounce kernels and the resolve.  ev (2*nb+2 events or
+// empty) brackets each bounce and the resolve.  With `staged`, each bounce's
+// grid is sized to the live rays of the previous one, read back by the host.
+int enqueue_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hipEvent_t> &ev, bool staged,
+                 uint32_t n_gen) {
+    HIPCHK(pcr_launch_reset(c->ctrs, 2 * sh.nb + 2, c->stream));
+    HIPCHK(pcr_launch_reset((uint32_t *)c->slot_counters, kSlots * 4 * 2, c->stream));
+    const int eb = pcr_bounce_block(sh.stack, sh.block);
+    for (int b = 0; b < sh.nb; b++) {
+        BounceArgs a = bounce_args(c, sh, b);
+        int grid = sh.grid;
+        if (staged) {
+            uint64_t live = n_gen;
+            if (b > 0) {
+                uint32_t cnt = 0;
+                HIPCHK(hipMemcpyAsync(&cnt, c->ctrs + 2 * (b - 1), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                live = std::min<uint64_t>(cnt, a.capacity);
+            }
+            if (live == 0) break;  // every path has ended
+            grid = (int)std::min<uint64_t>((live + eb - 1) / eb, 1u << 30);
+        }
+        if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
+        HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, grid, sh.block, c->stream));
+        if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
+    }
+    ResolveArgs ra{c->jobs_dev, c->rsegs_dev, c->pass_dev};
+    if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb], c->stream));
+    HIPCHK(pcr_launch_resolve(&ra, sh.rgrid, c->stream));
+    if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb + 1], c->stream));
+    return PCRHIP_OK;
+}
+
+int make_events(std::vector<hipEvent_t> &ev, int n) {
+    ev.assign(n, nullptr);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    return PCRHIP_OK;
+}
+
+// The captured graph of a pass shape, built on first use.
+int pass_graph(pcrhip_ctx *c, const PassShape &sh, pcrhip_ctx::PassGraph **out) {
+    for (auto &g : c->graphs)
+        if (g.nb == sh.nb && g.stack == sh.stack && g.block == sh.block && g.grid == sh.grid) {
+            *out = &g;
+            return PCRHIP_OK;
+        }
+    pcrhip_ctx::PassGraph g{sh.nb, sh.stack, sh.block, sh.grid, nullptr};
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    int rc = enqueue_pass(c, sh, {}, false, 0);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : fail(PCRHIP_EDEVICE, std::string("pass capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(PCRHIP_EDEVICE, std::string("pass graph instantiate: ") + hipGetErrorString(e));
+    c->graphs.push_back(g);
+    *out = &c->graphs.back();
+    return PCRHIP_OK;
+}
+
 // Runs one pass over every active job.  Returns a status; on a device error the
 // context is unusable and every job fails.
-int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs) {
+int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config &cfg) {
     std::vector<GenSeg> segs;
     std::vector<ResolveSeg> rsegs;
     uint64_t n_gen = 0, n_pix = 0;
     int max_depth = 0, stack = 8;
-    std::vector<int32_t> slots;
     for (auto *j : jobs) {
         uint64_t pix = (uint64_t)j->cw * j->ch;
         int spp_pass = (int)std::max<uint64_t>(1, std::min<uint64_t>(j->d.spp - j->s_next, j->rad_paths / pix));
@@ -295,49 +393,34 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs) {
         n_pix += pix;
         max_depth = std::max(max_depth, j->d.depth);
         stack = std::max(stack, j->scene->stack);
-        slots.push_back(j->slot);
     }
     if (n_gen >= (1ull << 31)) return fail(PCRHIP_EINVAL, "pass too large");
     int rc = ensure_queues(c, 4 * n_gen);  // a dispersion fork emits up to 4 rays per path
     if (rc) return rc;
+    *c->pass_host = PassInfo{(uint32_t)n_gen, (int32_t)segs.size(), (uint32_t)n_pix, (int32_t)rsegs.size()};
     for (auto *j : jobs)
         HIPCHK(hipMemcpyAsync(c->jobs_dev + j->slot, c->jobs_host + j->slot, sizeof(DJob), hipMemcpyHostToDevice,
                               c->stream));
     HIPCHK(hipMemcpyAsync(c->segs_dev, segs.data(), segs.size() * sizeof(GenSeg), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->rsegs_dev, rsegs.data(), rsegs.size() * sizeof(ResolveSeg), hipMemcpyHostToDevice,
                           c->stream));
-    int nb = std::max(max_depth, 1);
-    HIPCHK(pcr_launch_reset(c->ctrs, 2 * nb + 2, c->stream));
-    HIPCHK(hipMemsetAsync(c->slot_counters, 0, sizeof(unsigned long long) * kSlots * 4, c->stream));
+    HIPCHK(hipMemcpyAsync(c->pass_dev, c->pass_host, sizeof(PassInfo), hipMemcpyHostToDevice, c->stream));
+    const PassShape sh{std::max(max_depth, 1), stack, cfg.block, c->n_cu * std::max(1, cfg.blocks_per_cu),
+                       c->n_cu * 4};
+    const int nb = sh.nb;
+    // Per-kernel event timing needs the launches on the stream itself, so a
+    // profiled pass of the graph schedule is issued as the same persistent
+    // launches (same kernels, same grid).
     const bool prof = c->profile.load();
     std::vector<hipEvent_t> ev;
-    if (prof) {
-        ev.resize(2 * nb + 2);
-        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    if (cfg.schedule == PCRHIP_SCHED_GRAPH && !prof) {
+        pcrhip_ctx::PassGraph *g = nullptr;
+        if ((rc = pass_graph(c, sh, &g))) return rc;
+        HIPCHK(hipGraphLaunch(g->exec, c->stream));
+    } else {
+        if (prof && (rc = make_events(ev, 2 * nb + 2))) return rc;
+        if ((rc = enqueue_pass(c, sh, ev, cfg.schedule == PCRHIP_SCHED_STAGED, (uint32_t)n_gen))) return rc;
     }
-    const int grid = c->n_cu * std::max(1, c->cfg.blocks_per_cu);
-    for (int b = 0; b < nb; b++) {
-        BounceArgs a{};
-        a.jobs = c->jobs_dev;
-        a.segs = c->segs_dev;
-        a.n_segs = (int)segs.size();
-        a.n_gen = (uint32_t)n_gen;
-        a.in = (b & 1) ? c->qa : c->qb;
-        a.out = (b & 1) ? c->qb : c->qa;
-        a.count_in = b == 0 ? nullptr : c->ctrs + 2 * (b - 1);
-        a.count_out = c->ctrs + 2 * b;
-        a.work = c->ctrs + 2 * b + 1;
-        a.capacity = (uint32_t)std::min<uint64_t>(c->qcap, 0xFFFFFFFFu);
-        a.overflow = c->ctrs + 2 * nb;
-        a.counters = c->slot_counters;
-        if (prof) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
-        HIPCHK(pcr_launch_bounce(&a, b == 0, stack, grid, c->cfg.block, c->stream));
-        if (prof) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
-    }
-    ResolveArgs ra{c->jobs_dev, c->rsegs_dev, (int)rsegs.size()};
-    if (prof) HIPCHK(hipEventRecord(ev[2 * nb], c->stream));
-    HIPCHK(pcr_launch_resolve(&ra, (uint32_t)n_pix, c->stream));
-    if (prof) HIPCHK(hipEventRecord(ev[2 * nb + 1], c->stream));
     std::vector<uint32_t> ctr(2 * nb + 2);
     HIPCHK(hipMemcpyAsync(ctr.data(), c->ctrs, ctr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     std::vector<unsigned long long> sc(kSlots * 4);
@@ -350,7 +433,8 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs) {
         uint64_t rays_in = n_gen;  // generating bounce reads no queue
         for (int b = 0; b < nb; b++) {
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]));
+            // a staged pass stops launching once every path has ended
+            if (hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]) != hipSuccess) break;
             c->prof.bounce_ms += ms;
             c->prof.bounce_launches++;
             uint64_t written = ctr[2 * b];
@@ -358,9 +442,10 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs) {
             rays_in = written;
         }
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, ev[2 * nb], ev[2 * nb + 1]));
-        c->prof.resolve_ms += ms;
-        c->prof.resolve_launches++;
+        if (hipEventElapsedTime(&ms, ev[2 * nb], ev[2 * nb + 1]) == hipSuccess) {
+            c->prof.resolve_ms += ms;
+            c->prof.resolve_launches++;
+        }
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
     for (auto *j : jobs) {
@@ -398,8 +483,9 @@ void worker_main(pcrhip_ctx *c) {
                 else run.push_back(j);
             }
         if (run.empty()) continue;
+        const pcrhip_config cfg = c->cfg;
         lk.unlock();
-        int rc = run_pass(c, run);
+        int rc = run_pass(c, run, cfg);
         std::string err = rc ? g_err : std::string();
         lk.lock();
         if (rc) {
@@ -664,6 +750,7 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
     c->cfg.target_pass_paths = 16u << 20;
     c->cfg.memory_budget = 64ull << 30;
     c->cfg.max_active_jobs = 32;
+    c->cfg.schedule = PCRHIP_SCHED_GRAPH;
     auto cleanup = [&](int rc) {
         (void)hipStreamDestroy(c->stream);
         (void)hipFree(c->jobs_dev);
@@ -671,7 +758,9 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
         (void)hipFree(c->rsegs_dev);
         (void)hipFree(c->ctrs);
         (void)hipFree(c->slot_counters);
+        (void)hipFree(c->pass_dev);
         (void)hipHostFree(c->jobs_host);
+        (void)hipHostFree(c->pass_host);
         delete c;
         return rc;
     };
@@ -682,7 +771,9 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
         (e = hipMalloc(&c->rsegs_dev, kMaxSegs * sizeof(ResolveSeg))) != hipSuccess ||
         (e = hipMalloc(&c->ctrs, (2 * kMaxDepthBounces + 2) * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&c->slot_counters, kSlots * 4 * sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipHostMalloc(&c->jobs_host, kSlots * sizeof(DJob), 0)) != hipSuccess)
+        (e = hipMalloc(&c->pass_dev, sizeof(PassInfo))) != hipSuccess ||
+        (e = hipHostMalloc(&c->jobs_host, kSlots * sizeof(DJob), 0)) != hipSuccess ||
+        (e = hipHostMalloc(&c->pass_host, sizeof(PassInfo), 0)) != hipSuccess)
         return cleanup(fail(PCRHIP_EDEVICE, std::string("context allocation failed: ") + hipGetErrorString(e)));
     std::memset(c->jobs_host, 0, kSlots * sizeof(DJob));
     c->worker = std::thread(worker_main, c);
@@ -699,8 +790,11 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     c->cv.notify_all();
     if (c->worker.joinable()) c->worker.join();
     (void)hipSetDevice(c->device);
+    drop_graphs(c);
     free_queue(c->qa);
     free_queue(c->qb);
+    (void)hipFree(c->pass_dev);
+    (void)hipHostFree(c->pass_host);
     (void)hipFree(c->jobs_dev);
     (void)hipFree(c->segs_dev);
     (void)hipFree(c->rsegs_dev);
@@ -724,6 +818,7 @@ int pcrhip_set_config(pcrhip_ctx *c, const pcrhip_config *cfg) {
     if (cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64) return fail(PCRHIP_EINVAL, "blocks_per_cu out of range");
     if (cfg->target_pass_paths < 1024) return fail(PCRHIP_EINVAL, "target_pass_paths too small");
     if (cfg->max_active_jobs < 1 || cfg->max_active_jobs > kSlots) return fail(PCRHIP_EINVAL, "max_active_jobs out of range");
+    if (cfg->schedule < PCRHIP_SCHED_GRAPH || cfg->schedule > PCRHIP_SCHED_STAGED) return fail(PCRHIP_EINVAL, "unknown schedule");
     std::lock_guard<std::mutex> lk(c->m);
     c->cfg = *cfg;
     return PCRHIP_OK;
@@ -937,6 +1032,85 @@ int pcrhip_render(pcrhip_ctx *c, pcrhip_scene *s, const pcrhip_job_desc *d, floa
     if (rc == PCRHIP_OK) rc = pcrhip_job_result(j, out, n, st);
     std::string err = rc ? g_err : std::string();
     pcrhip_job_release(j);
+    if (rc) g_err = err;
+    return rc;
+}
+
+int pcrhip_autotune(pcrhip_ctx *c, pcrhip_scene *s, const pcrhip_job_desc *d, uint64_t trial_paths,
+                    pcrhip_tune_entry *table, int max_entries, int *n_entries) {
+    if (!c || !s || !d) return fail(PCRHIP_EINVAL, "null argument");
+    int rc = validate_job(s, d);
+    if (rc) return rc;
+    pcrhip_config base;
+    {
+        std::lock_guard<std::mutex> lk(c->m);
+        if (!c->pending.empty()) return fail(PCRHIP_ESTATE, "autotune needs an idle context");
+        for (auto *j : c->active)
+            if (j) return fail(PCRHIP_ESTATE, "autotune needs an idle context");
+        base = c->cfg;
+    }
+    pcrhip_job_desc t = *d;
+    const uint64_t pix = (uint64_t)(d->cw ? d->cw : d->width) * (d->ch ? d->ch : d->height);
+    t.spp = (int)std::clamp<uint64_t>((trial_paths + pix / 2) / pix, 1, (uint64_t)d->spp);
+    std::vector<float> out(pix * 3);
+    std::vector<pcrhip_tune_entry> log;
+    auto trial = [&](const pcrhip_config &cfg, double &best_ms) -> int {
+        int r = pcrhip_set_config(c, &cfg);
+        if (r) return r;
+        best_ms = 1e300;
+        for (int rep = 0; rep < 3; rep++) {  // first repetition warms up (queues, graph capture)
+            auto t0 = Clock::now();
+            pcrhip_stats st{};
+            if ((r = pcrhip_render(c, s, &t, out.data(), out.size(), &st))) return r;
+            double ms = ms_between(t0, Clock::now());
+            if (rep > 0) best_ms = std::min(best_ms, ms);
+        }
+        log.push_back(pcrhip_tune_entry{cfg.block, cfg.blocks_per_cu, cfg.schedule, cfg.target_pass_paths, best_ms,
+                                        (double)pix * t.spp / best_ms / 1e3});
+        return PCRHIP_OK;
+    };
+    pcrhip_config best = base;
+    double best_ms = 1e300;
+    auto consider = [&](pcrhip_config cfg) -> int {
+        double ms = 0;
+        int r = trial(cfg, ms);
+        if (r) return r;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cfg;
+        }
+        return PCRHIP_OK;
+    };
+    // stage 1: block shape x persistent grid size
+    for (int blk : {64, 128, 256})
+        for (int bpc : {2, 4, 8, 16}) {
+            pcrhip_config cfg = base;
+            cfg.block = blk;
+            cfg.blocks_per_cu = bpc;
+            cfg.schedule = PCRHIP_SCHED_GRAPH;
+            if ((rc = consider(cfg))) break;
+        }
+    // stage 2: schedule
+    for (int sched : {PCRHIP_SCHED_PERSISTENT, PCRHIP_SCHED_STAGED}) {
+        if (rc) break;
+        pcrhip_config cfg = best;
+        cfg.schedule = sched;
+        rc = consider(cfg);
+    }
+    // stage 3: pass size (queue memory), up to twice the trial's paths
+    const uint64_t tp = pix * (uint64_t)t.spp;
+    for (uint64_t pp : {2ull << 20, 4ull << 20, 8ull << 20, 16ull << 20, 32ull << 20}) {
+        if (rc) break;
+        if (pp == best.target_pass_paths || pp > 2 * tp) continue;
+        pcrhip_config cfg = best;
+        cfg.target_pass_paths = pp;
+        rc = consider(cfg);
+    }
+    std::string err = rc ? g_err : std::string();
+    (void)pcrhip_set_config(c, rc ? &base : &best);
+    if (table)
+        for (int i = 0; i < (int)log.size() && i < max_entries; i++) table[i] = log[i];
+    if (n_entries) *n_entries = (int)log.size();
     if (rc) g_err = err;
     return rc;
 }

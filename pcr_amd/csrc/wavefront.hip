@@ -459,7 +459,8 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
     __shared__ uint32_t s_base;
     LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
     // An overflowing append still counts, so never read past the queue.
-    const uint32_t n_in = FIRST ? a.n_gen : min(*a.count_in, a.capacity);
+    const uint32_t n_in = FIRST ? a.pass->n_gen : min(*a.count_in, a.capacity);
+    const int n_segs = FIRST ? a.pass->n_segs : 0;
     LaneCounters lc;
     const uint32_t lane = lane_id();
     const ForkSink fork{&a.out, a.count_out, a.capacity, a.overflow};
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
         uint32_t gpath = 0;
         if (i < n_in) {
             if (FIRST) {
-                ok = gen_lookup(a.segs, a.n_segs, i, slot, gpath);
+                ok = gen_lookup(a.segs, n_segs, i, slot, gpath);
             } else {
                 st_load(a.in, i, ps);
                 slot = st_slot(ps.meta);
@@ -557,61 +558,62 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
 // CIE.h:140-159, averaged over the N hero channels) or RGB, added to the FP64
 // accumulator in sample order.
 __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    int slot = -1;
-    uint32_t pix = 0;
-    for (int k = 0; k < a.n_segs; k++) {
-        ResolveSeg g = a.segs[k];
-        if (item >= g.begin && item < g.begin + g.count) {
-            slot = g.slot;
-            pix = item - g.begin;
-            break;
-        }
-    }
-    if (slot < 0) return;
-    const DJob &J = a.jobs[slot];
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-    double *dst = J.accum + 3 * (size_t)pix;
-    acc0 = dst[0];
-    acc1 = dst[1];
-    acc2 = dst[2];
-    const bool spectral = (J.flags & JF_SPECTRAL) != 0;
-    const int N = J.hero;
+    const uint32_t n_items = a.pass->n_pix;
+    const int n_segs = a.pass->n_rsegs;
     const float sc = div_rn(300.f, kYbarIntegral);
-    for (int sl = 0; sl < J.spp_pass; sl++) {
-        uint32_t path = pix * (uint32_t)J.spp_pass + (uint32_t)sl;
-        float4 r = J.radiance[path];
-        float o0, o1, o2;
-        if (spectral) {
-            int px, py, s;
-            path_pixel(J, path, px, py, s);
-            uint64_t rng = path_stream(J, px, py, s);
-            if (J.flags & JF_JITTER) { rng_next(rng); rng_next(rng); }
-            if (J.flags & JF_STRATIFIED) { rng_next(rng); rng_next(rng); }
-            float l0 = fadd(400.f, fmul(rng_next(rng), 300.f));
-            float rr[4] = {r.x, r.y, r.z, r.w};
-            float X = 0.f, Y = 0.f, Z = 0.f;
-            _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < N) {
-                V3 cmf = fm_cmf(hero_lambda(l0, c, N));
-                X = fadd(X, fmul(fmul(rr[c], cmf.x), sc));
-                Y = fadd(Y, fmul(fmul(rr[c], cmf.y), sc));
-                Z = fadd(Z, fmul(fmul(rr[c], cmf.z), sc));
+    // grid-stride: the launch shape does not depend on the pass
+    for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+        int slot = -1;
+        uint32_t pix = 0;
+        for (int k = 0; k < n_segs; k++) {
+            ResolveSeg g = a.segs[k];
+            if (item >= g.begin && item < g.begin + g.count) {
+                slot = g.slot;
+                pix = item - g.begin;
+                break;
             }
-            o0 = div_rn(X, (float)N);
-            o1 = div_rn(Y, (float)N);
-            o2 = div_rn(Z, (float)N);
-        } else {
-            o0 = r.x;
-            o1 = r.y;
-            o2 = r.z;
         }
-        acc0 += (double)o0;
-        acc1 += (double)o1;
-        acc2 += (double)o2;
+        if (slot < 0) continue;
+        const DJob &J = a.jobs[slot];
+        double *dst = J.accum + 3 * (size_t)pix;
+        double acc0 = dst[0], acc1 = dst[1], acc2 = dst[2];
+        const bool spectral = (J.flags & JF_SPECTRAL) != 0;
+        const int N = J.hero;
+        for (int sl = 0; sl < J.spp_pass; sl++) {
+            uint32_t path = pix * (uint32_t)J.spp_pass + (uint32_t)sl;
+            float4 r = J.radiance[path];
+            float o0, o1, o2;
+            if (spectral) {
+                int px, py, s;
+                path_pixel(J, path, px, py, s);
+                uint64_t rng = path_stream(J, px, py, s);
+                if (J.flags & JF_JITTER) { rng_next(rng); rng_next(rng); }
+                if (J.flags & JF_STRATIFIED) { rng_next(rng); rng_next(rng); }
+                float l0 = fadd(400.f, fmul(rng_next(rng), 300.f));
+                float rr[4] = {r.x, r.y, r.z, r.w};
+                float X = 0.f, Y = 0.f, Z = 0.f;
+                _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < N) {
+                    V3 cmf = fm_cmf(hero_lambda(l0, c, N));
+                    X = fadd(X, fmul(fmul(rr[c], cmf.x), sc));
+                    Y = fadd(Y, fmul(fmul(rr[c], cmf.y), sc));
+                    Z = fadd(Z, fmul(fmul(rr[c], cmf.z), sc));
+                }
+                o0 = div_rn(X, (float)N);
+                o1 = div_rn(Y, (float)N);
+                o2 = div_rn(Z, (float)N);
+            } else {
+                o0 = r.x;
+                o1 = r.y;
+                o2 = r.z;
+            }
+            acc0 += (double)o0;
+            acc1 += (double)o1;
+            acc2 += (double)o2;
+        }
+        dst[0] = acc0;
+        dst[1] = acc1;
+        dst[2] = acc2;
     }
-    dst[0] = acc0;
-    dst[1] = acc1;
-    dst[2] = acc2;
 }
 
 // Closest hit for a fixed ray set (parity of hit ids with the reference).
@@ -654,15 +656,6 @@ __global__ void k_reset(uint32_t *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// Advances the pass index of every active job slot (graph replay state).
-__global__ void k_advance(DJob *jobs, const int32_t *slots, int n) {
-    int i = threadIdx.x;
-    if (i < n) {
-        DJob &J = jobs[slots[i]];
-        J.s_begin += J.spp_pass;
-    }
-}
-
 // ------------------------------------------------------------- launchers --
 template <bool FIRST, int STACK, int BLOCK>
 static hipError_t launch_bounce_t(const BounceArgs &a, int grid, hipStream_t s) {
@@ -678,17 +671,23 @@ static hipError_t launch_bounce_s(const BounceArgs &a, bool first, int grid, int
 
 }  // namespace pcr
 
+// LDS per block is STACK * BLOCK * 8 bytes: deep stacks use smaller blocks.
+extern "C" int pcr_bounce_block(int stack, int block) {
+    if (stack <= 16) return block;
+    if (stack <= 32) return block == 256 ? 128 : block;
+    return 64;
+}
 extern "C" hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block,
                                         hipStream_t s) {
     using namespace pcr;
+    block = pcr_bounce_block(stack, block);
     if (stack <= 8) return launch_bounce_s<8>(*a, first != 0, grid, block, s);
     if (stack <= 16) return launch_bounce_s<16>(*a, first != 0, grid, block, s);
-    if (stack <= 32) return launch_bounce_s<32>(*a, first != 0, grid, block == 256 ? 128 : block, s);
-    return launch_bounce_s<64>(*a, first != 0, grid, 64, s);
+    if (stack <= 32) return launch_bounce_s<32>(*a, first != 0, grid, block, s);
+    return launch_bounce_s<64>(*a, first != 0, grid, block, s);
 }
-extern "C" hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, uint32_t n_items, hipStream_t s) {
-    if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(pcr::k_resolve, dim3((n_items + 255) / 256), dim3(256), 0, s, *a);
+extern "C" hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(pcr::k_resolve, dim3(grid), dim3(256), 0, s, *a);
     return hipGetLastError();
 }
 extern "C" hipError_t pcr_launch_intersect(const pcr::IntersectArgs *a, int stack, hipStream_t s) {
@@ -701,9 +700,5 @@ extern "C" hipError_t pcr_launch_intersect(const pcr::IntersectArgs *a, int stac
 }
 extern "C" hipError_t pcr_launch_reset(uint32_t *p, int n, hipStream_t s) {
     hipLaunchKernelGGL(pcr::k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
-    return hipGetLastError();
-}
-extern "C" hipError_t pcr_launch_advance(pcr::DJob *jobs, const int32_t *slots, int n, hipStream_t s) {
-    hipLaunchKernelGGL(pcr::k_advance, dim3(1), dim3(64), 0, s, jobs, slots, n);
     return hipGetLastError();
 }

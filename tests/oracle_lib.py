@@ -216,3 +216,12 @@ def xyz_to_srgb(xyz):
     M = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
                   [0.0556434, -0.2040259, 1.0572252]])
     return np.asarray(xyz, np.float64) @ M.T
+
+
+def spread_seed(i):
+    """The i-th of a set of independent NumGen seeds.  Seeds that differ only
+    in their low 27 bits give every stream the same first draw (NumGen's
+    output is taken from the seeded state itself, NumGen.cpp:24-48, and
+    XSH-RR ignores those bits), so statistical tests spread their seeds over
+    all 64 bits."""
+    return ((i + 1) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF

@@ -1,0 +1,118 @@
+"""Pass schedules and the autotuner.
+
+The three schedules (a pass captured once in a hipGraph and replayed, the same
+persistent launches issued one by one, and staged launches sized to each
+bounce's live rays) must give the same FP64 sums bit for bit -- equal to the
+CPU oracle's -- across many passes, mixed jobs and graph replays interleaved
+with other device work.  The autotuner must leave a valid configuration that
+still renders bit-exactly."""
+import os
+
+import numpy as np
+import pytest
+
+import pcr_amd
+import scene_io
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+SC = os.path.join(HERE, "..", "pcr_amd", "scenes")
+
+
+@pytest.fixture
+def ctx_cfg(gpu_ctx):
+    saved = gpu_ctx.config()
+    yield gpu_ctx
+    gpu_ctx.set_config(**{f: getattr(saved, f) for f, _ in pcr_amd.Config._fields_})
+
+
+def oracle(path):
+    return scene_io.build_oracle_scene(scene_io.parse_scene(path))
+
+
+def render_sums(ds, **kw):
+    j = ds.submit(**kw)
+    j.wait()
+    s = j.sums()
+    _, st = j.result()
+    j.release()
+    return s, st
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("cornell-c1", dict(width=40, height=40, depth=8, spp=6, shadow=4)),
+    ("cornell-c2", dict(width=24, height=24, depth=5, spp=3, shadow=2)),
+    ("closed-c4", dict(width=32, height=32, depth=8, spp=4, shadow=2, mis=True, russian=True)),
+])
+def test_schedules_bit_identical(ctx_cfg, name, kw):
+    path = os.path.join(SC, name + ".json")
+    ds = ctx_cfg.upload(pcr_amd.HostScene(path))
+    want, _ = oracle(path).render_paths(kw["width"], kw["height"], kw["depth"], kw["spp"], kw["shadow"], 4242,
+                                        **{k: v for k, v in kw.items() if k in ("mis", "russian")})
+    try:
+        for sched in (pcr_amd.SCHED_GRAPH, pcr_amd.SCHED_PERSISTENT, pcr_amd.SCHED_STAGED, pcr_amd.SCHED_GRAPH):
+            # a pass of one sample row: many passes, so the graph is replayed many times
+            ctx_cfg.set_config(schedule=sched, target_pass_paths=max(1024, kw["width"] * kw["height"]))
+            got, st = render_sums(ds, seed=4242, **kw)
+            assert st.passes == kw["spp"]
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), sched
+            # other device work between replays
+            ds.intersect(np.array([[277.5, 277.5, 800.0, 0.0, 0.0, -1.0]], np.float32))
+    finally:
+        ds.release()
+
+
+def test_graph_mixed_jobs(ctx_cfg):
+    """Jobs of different scenes and depths share replayed pass graphs."""
+    ctx_cfg.set_config(schedule=pcr_amd.SCHED_GRAPH, target_pass_paths=1024)
+    specs = [("cornell-c1", dict(width=24, height=24, depth=8, spp=5, shadow=2, seed=11)),
+             ("cornell-c0", dict(width=20, height=20, depth=3, spp=7, shadow=3, seed=12)),
+             ("closed-c3", dict(width=16, height=16, depth=6, spp=4, shadow=1, seed=13, spectral=False))]
+    scenes = [ctx_cfg.upload(pcr_amd.HostScene(os.path.join(SC, n + ".json"))) for n, _ in specs]
+    try:
+        jobs = [ds.submit(**kw) for ds, (_, kw) in zip(scenes, specs)]
+        for (n, kw), j in zip(specs, jobs):
+            j.wait()
+            got = j.sums()
+            j.release()
+            k = dict(kw)
+            want, _ = oracle(os.path.join(SC, n + ".json")).render_paths(
+                k.pop("width"), k.pop("height"), k.pop("depth"), k.pop("spp"), k.pop("shadow"), k.pop("seed"), **k)
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), n
+    finally:
+        for ds in scenes:
+            ds.release()
+
+
+def test_graph_profile_events(ctx_cfg):
+    """Profiling times every kernel with events; under the graph schedule the
+    profiled passes are issued as direct launches of the same kernels."""
+    ctx_cfg.set_config(schedule=pcr_amd.SCHED_GRAPH)
+    ds = ctx_cfg.upload(pcr_amd.HostScene(os.path.join(SC, "cornell-c1.json")))
+    try:
+        ctx_cfg.profile(enable=True, reset=True)
+        render_sums(ds, width=64, height=64, depth=4, spp=8, shadow=2, seed=5)
+        p = ctx_cfg.profile(enable=False, reset=True)
+        assert p.bounce_launches >= 4 and p.bounce_ms > 0.0 and p.resolve_launches >= 1
+    finally:
+        ctx_cfg.profile(enable=False, reset=True)
+        ds.release()
+
+
+def test_autotune(ctx_cfg):
+    path = os.path.join(SC, "cornell-c1.json")
+    ds = ctx_cfg.upload(pcr_amd.HostScene(path))
+    try:
+        kw = dict(width=64, height=64, depth=6, spp=64, shadow=2)
+        table = ds.autotune(trial_paths=64 * 64 * 16, seed=1, **kw)
+        assert len(table) >= 14
+        assert all(t["ms"] > 0 and t["mpaths_s"] > 0 for t in table)
+        cfg = ctx_cfg.config()
+        best = min(table, key=lambda t: t["ms"])
+        assert (cfg.block, cfg.blocks_per_cu, cfg.schedule, cfg.target_pass_paths) == \
+            (best["block"], best["blocks_per_cu"], best["schedule"], best["target_pass_paths"])
+        got, _ = render_sums(ds, seed=77, **kw)
+        want, _ = oracle(path).render_paths(64, 64, 6, 64, 2, 77)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    finally:
+        ds.release()

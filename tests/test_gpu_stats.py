@@ -16,6 +16,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle_lib as ol
 import pcr_amd
 import scene_io
 
@@ -43,11 +44,11 @@ def test_relmse_and_means_vs_cpu(gpu_ctx, golden, key):
     try:
         gpu, cpu = [], []
         for k in range(K):
-            j = ds.submit(width=W, height=H, depth=d, spp=spp, shadow=S, seed=100 + k, crop=(x0, y0, cw, ch))
+            j = ds.submit(width=W, height=H, depth=d, spp=spp, shadow=S, seed=ol.spread_seed(100 + k), crop=(x0, y0, cw, ch))
             j.wait()
             gpu.append(j.sums() / spp)
             j.release()
-            acc, _ = osc.render_paths(W, H, d, spp, S, 900 + k, crop=(x0, y0, cw, ch))
+            acc, _ = osc.render_paths(W, H, d, spp, S, ol.spread_seed(900 + k), crop=(x0, y0, cw, ch))
             cpu.append(acc / spp)
     finally:
         ds.release()

@@ -152,7 +152,7 @@ def test_direct_lighting_closed_form():
     om = np.array([rect_solid_angle(float(p[0]), float(p[2]), x0, x1, z0, z1, h) for p in hits["hit"]])
     want = (rho / math.pi * Le * h * om).reshape(ch, cw)
     K, spp = 32, 128
-    est = np.stack([sc.render_paths(W, H, 1, spp, 4, 1000 + k, spectral=False, crop=crop)[0][..., 0] / spp
+    est = np.stack([sc.render_paths(W, H, 1, spp, 4, ol.spread_seed(1000 + k), spectral=False, crop=crop)[0][..., 0] / spp
                     for k in range(K)])
     mean, se = est.mean(0), est.std(0, ddof=1) / math.sqrt(K)
     z = (mean - want) / se
@@ -161,7 +161,7 @@ def test_direct_lighting_closed_form():
     zt = (tot.mean() - want.mean()) / (tot.std(ddof=1) / math.sqrt(K))
     assert abs(zt) < 3.0, zt
     # the branched estimator: one path per pixel with 32 shadow rays, 64 seeds
-    br = np.stack([sc.render_branched(W, H, 1, 1, 32, 3000 + k)[0][cy:cy + ch, cx:cx + cw, 0] for k in range(64)])
+    br = np.stack([sc.render_branched(W, H, 1, 1, 32, ol.spread_seed(3000 + k))[0][cy:cy + ch, cx:cx + cw, 0] for k in range(64)])
     zb = (br.mean() - want.mean()) / (br.mean(axis=(1, 2)).std(ddof=1) / 8.0)
     assert abs(zb) < 3.0, zb
 
@@ -186,14 +186,14 @@ def test_per_path_mean_equals_branched_mean(case):
     name, depth, samples, spp, flags = MEAN_CASES[case]
     sc = scene(name)
     W = H = 48
-    K = 8
+    K = 24
     spectral = flags.get("spectral", False)
     b = []
     p = []
     for k in range(K):
-        lin, _ = sc.render_branched(W, H, depth, samples, 4, 5000 + k, **flags)
+        lin, _ = sc.render_branched(W, H, depth, samples, 4, ol.spread_seed(5000 + k), **flags)
         b.append(lin.reshape(-1, 3).astype(np.float64).mean(0))
-        acc, _ = sc.render_paths(W, H, depth, spp, 4, 7000 + k, **flags)
+        acc, _ = sc.render_paths(W, H, depth, spp, 4, ol.spread_seed(7000 + k), **flags)
         m = acc.reshape(-1, 3).mean(0) / spp
         p.append(ol.xyz_to_srgb(m) if spectral else m)
     b, p = np.array(b), np.array(p)
