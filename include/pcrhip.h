@@ -238,6 +238,9 @@ typedef struct {
     uint64_t resolve_launches;
     double resolve_ms;
     uint64_t queue_bytes;   /* bytes of SoA queue records read + written */
+    uint64_t passes;        /* passes run */
+    uint64_t pass_paths;    /* paths generated by those passes */
+    double pass_wall_ms;    /* host wall time spent in those passes, launch to results */
 } pcrhip_profile;
 int pcrhip_profile_get(pcrhip_ctx *ctx, pcrhip_profile *out, int reset);
 

@@ -74,7 +74,8 @@ class Stats(C.Structure):
 
 class Profile(C.Structure):
     _fields_ = [("bounce_launches", C.c_uint64), ("bounce_ms", C.c_double), ("resolve_launches", C.c_uint64),
-                ("resolve_ms", C.c_double), ("queue_bytes", C.c_uint64)]
+                ("resolve_ms", C.c_double), ("queue_bytes", C.c_uint64), ("passes", C.c_uint64),
+                ("pass_paths", C.c_uint64), ("pass_wall_ms", C.c_double)]
 
 
 # every entry point of include/pcrhip.h: name -> (restype, argtypes)

@@ -65,11 +65,10 @@ struct BounceArgs {
     const GenSeg *segs;
     const PassInfo *pass;
     QueueView in, out;
-    const uint32_t *count_in;     // live rays in `in`
-    uint32_t *count_out;          // rays appended to `out`
+    const uint32_t *count_in;     // [kMaxSegs] live rays per segment in `in`
+    uint32_t *count_out;          // [kMaxSegs] rays appended per segment to `out`
     uint32_t *work;               // work-stealing counter of this launch
-    uint32_t capacity;            // rays `out` can hold
-    uint32_t *overflow;           // set if an append would exceed capacity
+    uint32_t *overflow;           // set if an append would exceed its region
     unsigned long long *counters; // paths, extension rays, shadow rays, emissive hits
 };
 

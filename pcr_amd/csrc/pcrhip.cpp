@@ -72,6 +72,7 @@ float ibits(int32_t v) {
 
 constexpr int kSlots = 64;
 constexpr int kMaxDepthBounces = 255;
+constexpr int kCtrStride = kMaxSegs + 1;  // counters of one bounce
 
 }  // namespace
 
@@ -119,7 +120,7 @@ struct pcrhip_ctx {
     DJob *jobs_dev = nullptr;
     GenSeg *segs_dev = nullptr;
     ResolveSeg *rsegs_dev = nullptr;
-    uint32_t *ctrs = nullptr;  // [2 * kMaxDepthBounces + 2]: per bounce (count_out, work), overflow
+    uint32_t *ctrs = nullptr;  // per bounce: kMaxSegs live-ray counts + work counter; then overflow
     unsigned long long *slot_counters = nullptr;  // [kSlots * 4]
     PassInfo *pass_dev = nullptr;
     QueueView qa{}, qb{};
@@ -302,11 +303,10 @@ BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
     a.pass = c->pass_dev;
     a.in = (b & 1) ? c->qa : c->qb;
     a.out = (b & 1) ? c->qb : c->qa;
-    a.count_in = b == 0 ? nullptr : c->ctrs + 2 * (b - 1);
-    a.count_out = c->ctrs + 2 * b;
-    a.work = c->ctrs + 2 * b + 1;
-    a.capacity = (uint32_t)std::min<uint64_t>(c->qcap, 0xFFFFFFFFu);
-    a.overflow = c->ctrs + 2 * sh.nb;
+    a.count_in = b == 0 ? nullptr : c->ctrs + kCtrStride * (b - 1);
+    a.count_out = c->ctrs + kCtrStride * b;
+    a.work = c->ctrs + kCtrStride * b + kMaxSegs;
+    a.overflow = c->ctrs + kCtrStride * sh.nb;
     a.counters = c->slot_counters;
     return a;
 }
@@ -316,8 +316,8 @@ BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
 // empty) brackets each bounce and the resolve.  With `staged`, each bounce's
 // grid is sized to the live rays of the previous one, read back by the host.
 int enqueue_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hipEvent_t> &ev, bool staged,
-                 uint32_t n_gen) {
-    HIPCHK(pcr_launch_reset(c->ctrs, 2 * sh.nb + 2, c->stream));
+                 uint32_t n_gen, const std::vector<GenSeg> &segs) {
+    HIPCHK(pcr_launch_reset(c->ctrs, kCtrStride * sh.nb + 1, c->stream));
     HIPCHK(pcr_launch_reset((uint32_t *)c->slot_counters, kSlots * 4 * 2, c->stream));
     const int eb = pcr_bounce_block(sh.stack, sh.block);
     for (int b = 0; b < sh.nb; b++) {
@@ -326,10 +326,12 @@ int enqueue_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hipEvent_
         if (staged) {
             uint64_t live = n_gen;
             if (b > 0) {
-                uint32_t cnt = 0;
-                HIPCHK(hipMemcpyAsync(&cnt, c->ctrs + 2 * (b - 1), sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+                uint32_t cnt[kMaxSegs];
+                HIPCHK(hipMemcpyAsync(cnt, a.count_in, segs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                      c->stream));
                 HIPCHK(hipStreamSynchronize(c->stream));
-                live = std::min<uint64_t>(cnt, a.capacity);
+                live = 0;
+                for (size_t k = 0; k < segs.size(); k++) live += std::min<uint64_t>(cnt[k], 4ull * segs[k].count);
             }
             if (live == 0) break;  // every path has ended
             grid = (int)std::min<uint64_t>((live + eb - 1) / eb, 1u << 30);
@@ -361,7 +363,7 @@ int pass_graph(pcrhip_ctx *c, const PassShape &sh, pcrhip_ctx::PassGraph **out) 
     pcrhip_ctx::PassGraph g{sh.nb, sh.stack, sh.block, sh.grid, nullptr};
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    int rc = enqueue_pass(c, sh, {}, false, 0);
+    int rc = enqueue_pass(c, sh, {}, false, 0, {});
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc || e != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -378,13 +380,50 @@ int pass_graph(pcrhip_ctx *c, const PassShape &sh, pcrhip_ctx::PassGraph **out) 
 // Runs one pass over every active job.  Returns a status; on a device error the
 // context is unusable and every job fails.
 int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config &cfg) {
+    const auto t_pass = Clock::now();
     std::vector<GenSeg> segs;
     std::vector<ResolveSeg> rsegs;
     uint64_t n_gen = 0, n_pix = 0;
     int max_depth = 0, stack = 8;
-    for (auto *j : jobs) {
+    // Paths per job in this pass: the pass budget water-filled over the jobs
+    // (a job that needs less than an equal share takes what it needs and the
+    // rest is shared by the others), at least one sample per pixel, at most
+    // what the job's radiance buffer holds.  Small jobs therefore finish in few
+    // passes while large ones progress at an equal rate.
+    const size_t nj = jobs.size();
+    std::vector<uint64_t> want(nj), alloc(nj, 0);
+    for (size_t i = 0; i < nj; i++) {
+        const pcrhip_job *j = jobs[i];
         uint64_t pix = (uint64_t)j->cw * j->ch;
-        int spp_pass = (int)std::max<uint64_t>(1, std::min<uint64_t>(j->d.spp - j->s_next, j->rad_paths / pix));
+        want[i] = std::min<uint64_t>((uint64_t)(j->d.spp - j->s_next) * pix, j->rad_paths);
+    }
+    {
+        uint64_t left = std::max<uint64_t>(cfg.target_pass_paths, 1);
+        std::vector<size_t> open(nj);
+        for (size_t i = 0; i < nj; i++) open[i] = i;
+        while (!open.empty()) {
+            uint64_t share = left / open.size();
+            std::vector<size_t> rest;
+            for (size_t i : open) {
+                if (want[i] <= share) {
+                    alloc[i] = want[i];
+                    left -= want[i];
+                } else {
+                    rest.push_back(i);
+                }
+            }
+            if (rest.size() == open.size()) {
+                for (size_t i : rest) alloc[i] = share;
+                break;
+            }
+            open.swap(rest);
+        }
+    }
+    for (size_t i = 0; i < nj; i++) {
+        pcrhip_job *j = jobs[i];
+        uint64_t pix = (uint64_t)j->cw * j->ch;
+        int spp_pass = (int)std::clamp<uint64_t>(alloc[i] / pix, 1,
+                                                 std::min<uint64_t>(j->d.spp - j->s_next, j->rad_paths / pix));
         j->spp_pass = spp_pass;
         fill_djob(j, c->jobs_host[j->slot]);
         segs.push_back(GenSeg{j->slot, (uint32_t)n_gen, (uint32_t)(pix * spp_pass), 0u});
@@ -394,7 +433,8 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         max_depth = std::max(max_depth, j->d.depth);
         stack = std::max(stack, j->scene->stack);
     }
-    if (n_gen >= (1ull << 31)) return fail(PCRHIP_EINVAL, "pass too large");
+    // queue regions are 4 rays per path, addressed with 32-bit indices
+    if (4 * n_gen >= (1ull << 32)) return fail(PCRHIP_EINVAL, "pass too large");
     int rc = ensure_queues(c, 4 * n_gen);  // a dispersion fork emits up to 4 rays per path
     if (rc) return rc;
     *c->pass_host = PassInfo{(uint32_t)n_gen, (int32_t)segs.size(), (uint32_t)n_pix, (int32_t)rsegs.size()};
@@ -419,15 +459,15 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         HIPCHK(hipGraphLaunch(g->exec, c->stream));
     } else {
         if (prof && (rc = make_events(ev, 2 * nb + 2))) return rc;
-        if ((rc = enqueue_pass(c, sh, ev, cfg.schedule == PCRHIP_SCHED_STAGED, (uint32_t)n_gen))) return rc;
+        if ((rc = enqueue_pass(c, sh, ev, cfg.schedule == PCRHIP_SCHED_STAGED, (uint32_t)n_gen, segs))) return rc;
     }
-    std::vector<uint32_t> ctr(2 * nb + 2);
+    std::vector<uint32_t> ctr(kCtrStride * nb + 1);
     HIPCHK(hipMemcpyAsync(ctr.data(), c->ctrs, ctr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     std::vector<unsigned long long> sc(kSlots * 4);
     HIPCHK(hipMemcpyAsync(sc.data(), c->slot_counters, sc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (ctr[2 * nb]) return fail(PCRHIP_EDEVICE, "ray queue overflow");
+    if (ctr[kCtrStride * nb]) return fail(PCRHIP_EDEVICE, "ray queue overflow");
     if (prof) {
         std::lock_guard<std::mutex> lk(c->pm);
         uint64_t rays_in = n_gen;  // generating bounce reads no queue
@@ -437,7 +477,8 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
             if (hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]) != hipSuccess) break;
             c->prof.bounce_ms += ms;
             c->prof.bounce_launches++;
-            uint64_t written = ctr[2 * b];
+            uint64_t written = 0;
+            for (size_t k = 0; k < segs.size(); k++) written += ctr[kCtrStride * b + k];
             c->prof.queue_bytes += (b == 0 ? 0 : rays_in * 80) + written * 80;
             rays_in = written;
         }
@@ -447,6 +488,9 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
             c->prof.resolve_launches++;
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
+        c->prof.passes++;
+        c->prof.pass_paths += n_gen;
+        c->prof.pass_wall_ms += ms_between(t_pass, Clock::now());
     }
     for (auto *j : jobs) {
         for (int k = 0; k < 4; k++) j->counters[k] += sc[j->slot * 4 + k];
@@ -769,7 +813,7 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
         (e = hipMalloc(&c->jobs_dev, kSlots * sizeof(DJob))) != hipSuccess ||
         (e = hipMalloc(&c->segs_dev, kMaxSegs * sizeof(GenSeg))) != hipSuccess ||
         (e = hipMalloc(&c->rsegs_dev, kMaxSegs * sizeof(ResolveSeg))) != hipSuccess ||
-        (e = hipMalloc(&c->ctrs, (2 * kMaxDepthBounces + 2) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&c->ctrs, (kCtrStride * kMaxDepthBounces + 1) * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&c->slot_counters, kSlots * 4 * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc(&c->pass_dev, sizeof(PassInfo))) != hipSuccess ||
         (e = hipHostMalloc(&c->jobs_host, kSlots * sizeof(DJob), 0)) != hipSuccess ||

@@ -194,12 +194,14 @@ struct BounceCounters {
     uint32_t ext, shadow, emit;
 };
 
-// Appends the sub-paths of a dispersion fork straight to the output queue with
-// one atomic for the lane (forks are rare: one glass hit per path at most).
+// Appends the sub-paths of a dispersion fork straight to the output queue
+// region of the path's job with one atomic for the lane (forks are rare: one
+// glass hit per path at most).
 struct ForkSink {
     const QueueView *out;
-    uint32_t *count;
-    uint32_t capacity;
+    uint32_t *count;     // rays appended to the job's region
+    uint32_t region;     // first queue index of the region
+    uint32_t capacity;   // rays the region holds
     uint32_t *overflow;
 };
 
@@ -270,8 +272,20 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
         if (spectral && cB != 0.f && nch > 1) {
             // Dispersion fork: one single-wavelength sub-path per hero channel,
             // each on its own stream derived from the parent's state.
+            // One atomic per wave for all forking lanes: the lanes of a round
+            // belong to one job, and a path forks only while it still carries
+            // all of the job's hero channels, so nch is the same on every
+            // forking lane.  (Per-lane atomics on the one counter serialise.)
             uint32_t base = 0;
-            if (nb < J.depth) base = atomicAdd(fork.count, (uint32_t)nch);
+            if (nb < J.depth) {
+                const uint64_t act = __ballot(1);
+                const int leader = __ffsll((unsigned long long)act) - 1;
+                const uint32_t ln = __lane_id();
+                uint32_t wb = 0;
+                if ((int)ln == leader) wb = atomicAdd(fork.count, (uint32_t)(__popcll(act) * nch));
+                wb = __shfl(wb, leader, 64);
+                base = wb + (uint32_t)__popcll(act & ((1ull << ln) - 1ull)) * (uint32_t)nch;
+            }
             _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) {
                 PathState ch = ps;
                 ch.rng = stream_seed(ps.rng, (uint64_t)(c + 1));
@@ -287,7 +301,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
                 if (nb >= J.depth) {
                     write_radiance(J, ch);
                 } else if (base + (uint32_t)c < fork.capacity) {
-                    st_store(*fork.out, base + (uint32_t)c, ch);
+                    st_store(*fork.out, fork.region + base + (uint32_t)c, ch);
                 } else {
                     atomicOr(fork.overflow, 1u);
                 }
@@ -425,45 +439,85 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Work item -> (job slot, job-local path index) for the generating bounce.
-__device__ __forceinline__ bool gen_lookup(const GenSeg *segs, int nseg, uint32_t item, int &slot, uint32_t &path) {
-    for (int k = 0; k < nseg; k++) {
-        GenSeg g = segs[k];
-        if (item < g.begin + g.count && item >= g.begin) {
-            slot = g.slot;
-            path = g.path0 + (item - g.begin);
-            return true;
-        }
+// Work item -> segment: the largest k with pref[k] <= item (pref is the
+// exclusive prefix of the segments' item counts, in LDS; empty segments are
+// skipped because their successor has the same prefix).
+__device__ __forceinline__ int seg_of(const uint32_t *pref, int nseg, uint32_t item) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pref[mid] <= item) lo = mid;
+        else hi = mid - 1;
     }
-    return false;
+    return lo;
 }
 
-// Per-lane counters of the job slot the lane is currently processing; flushed
-// with atomics when the slot changes (only in mixed-job passes) and at exit.
+// Per-lane counts for the job slot the wave is currently working on.  When
+// the wave moves to another job (only in passes that mix jobs) and at exit,
+// the whole wave reduces them and one lane adds them to the slot's counters:
+// per-lane atomics on a handful of addresses would serialise.
 struct LaneCounters {
-    int slot = -1;
     uint32_t paths = 0, ext = 0, shadow = 0, emit = 0;
 };
-__device__ __forceinline__ void flush_lane(const LaneCounters &c, unsigned long long *ctr) {
-    unsigned long long *p = ctr + 4 * c.slot;
-    if (c.paths) atomicAdd(p + 0, (unsigned long long)c.paths);
-    if (c.ext) atomicAdd(p + 1, (unsigned long long)c.ext);
-    if (c.shadow) atomicAdd(p + 2, (unsigned long long)c.shadow);
-    if (c.emit) atomicAdd(p + 3, (unsigned long long)c.emit);
+// Needs every lane of the wave active.
+__device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsigned long long *ctr) {
+    uint32_t p = c.paths, e = c.ext, s = c.shadow, m = c.emit;
+    for (int dd = 32; dd >= 1; dd >>= 1) {
+        p += __shfl_xor(p, dd, 64);
+        e += __shfl_xor(e, dd, 64);
+        s += __shfl_xor(s, dd, 64);
+        m += __shfl_xor(m, dd, 64);
+    }
+    if (__lane_id() == 0) {
+        unsigned long long *q = ctr + 4 * slot;
+        if (p) atomicAdd(q + 0, (unsigned long long)p);
+        if (e) atomicAdd(q + 1, (unsigned long long)e);
+        if (s) atomicAdd(q + 2, (unsigned long long)s);
+        if (m) atomicAdd(q + 3, (unsigned long long)m);
+    }
 }
 
+// Queue layout: every job of the pass (segment k of the GenSeg table) owns the
+// region [4*begin_k, 4*(begin_k + count_k)) of both ping-pong queues -- room
+// for every path of the job to fork into its 4 hero channels -- and a live-ray
+// counter per bounce.  A launch walks the concatenation of the segments' live
+// rays in 128-ray chunks, so a wave almost always holds rays of one job: the
+// job's descriptor and scene are read with scalar loads and the shading code
+// runs once per wave.
 template <bool FIRST, int STACK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
     __shared__ int32_t s_idx[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
     __shared__ uint32_t s_base;
+    __shared__ uint32_t s_pref[kMaxSegs + 1];
+    __shared__ GenSeg s_segs[kMaxSegs];
     LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
-    // An overflowing append still counts, so never read past the queue.
-    const uint32_t n_in = FIRST ? a.pass->n_gen : min(*a.count_in, a.capacity);
-    const int n_segs = FIRST ? a.pass->n_segs : 0;
-    LaneCounters lc;
+    const int n_segs = a.pass->n_segs;
     const uint32_t lane = lane_id();
-    const ForkSink fork{&a.out, a.count_out, a.capacity, a.overflow};
+    // The segment table and the exclusive prefix of the segments' input
+    // counts, staged in LDS (one wave scans <= 64 segments): every ray's queue
+    // address depends on them, so they must not cost a memory round trip.
+    if (threadIdx.x < 64) {
+        const int k = (int)threadIdx.x;
+        uint32_t v = 0;
+        if (k < n_segs) {
+            const GenSeg g = a.segs[k];
+            s_segs[k] = g;
+            // an overflowing append still counts: never read past the region
+            v = FIRST ? g.count : min(a.count_in[k], 4u * g.count);
+        }
+        uint32_t incl = v;
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            uint32_t t = __shfl_up(incl, dd, 64);
+            if (k >= dd) incl += t;
+        }
+        if (k < n_segs) s_pref[k] = incl - v;
+        if (k == 63) s_pref[kMaxSegs] = incl;
+    }
+    __syncthreads();
+    const uint32_t n_in = s_pref[kMaxSegs];
+    LaneCounters lc;
+    int wslot = -1;  // job slot of the counts in lc (wave-uniform)
     for (;;) {
         if (threadIdx.x == 0) s_base = atomicAdd(a.work, (uint32_t)BLOCK);
         __syncthreads();
@@ -473,35 +527,41 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
         const uint32_t i = base + threadIdx.x;
         PathState ps;
         bool ok = false;
-        int slot = -1;
+        int seg = -1;
         uint32_t gpath = 0;
         if (i < n_in) {
+            seg = n_segs == 1 ? 0 : seg_of(s_pref, n_segs, i);
+            const uint32_t r = i - s_pref[seg];
             if (FIRST) {
-                ok = gen_lookup(a.segs, n_segs, i, slot, gpath);
+                gpath = s_segs[seg].path0 + r;
             } else {
-                st_load(a.in, i, ps);
-                slot = st_slot(ps.meta);
-                ok = true;
+                st_load(a.in, 4u * s_segs[seg].begin + r, ps);
             }
+            ok = true;
         }
-        // Waterfall over the job slots present in the wave: each round takes
-        // the first pending lane's slot as a wave-uniform value, so the job
-        // descriptor and scene are read with scalar loads.  A wave of a single
-        // job (the common case) runs one round.
+        // Waterfall over the segments (jobs) present in the wave: each round
+        // takes the first pending lane's segment as a wave-uniform value, so
+        // the job descriptor and scene are read with scalar loads.  Chunks of
+        // one segment make a single round the common case.
         bool alive = false;
         bool pending = ok;
         while (__ballot(pending)) {
             const uint64_t pm = __ballot(pending);
-            const int s0 = __builtin_amdgcn_readfirstlane(__shfl(slot, __ffsll((unsigned long long)pm) - 1, 64));
-            if (pending && slot == s0) {
+            const int k0 = __builtin_amdgcn_readfirstlane(__shfl(seg, __ffsll((unsigned long long)pm) - 1, 64));
+            // wave-uniform values (readfirstlane): scalar loads of the job
+            const int s0 = __builtin_amdgcn_readfirstlane(s_segs[k0].slot);
+            if (s0 != wslot) {  // every lane is active here
+                if (wslot >= 0) wave_flush(lc, wslot, a.counters);
+                lc = LaneCounters{};
+                wslot = s0;
+            }
+            if (pending && seg == k0) {
                 pending = false;
-                if (s0 != lc.slot) {
-                    if (lc.slot >= 0) flush_lane(lc, a.counters);
-                    lc = LaneCounters{};
-                    lc.slot = s0;
-                }
+                const uint32_t g_begin = __builtin_amdgcn_readfirstlane(s_segs[k0].begin);
+                const uint32_t g_count = __builtin_amdgcn_readfirstlane(s_segs[k0].count);
                 const DJob &J = a.jobs[s0];
                 if (FIRST) gen_path(J, s0, gpath, ps);
+                const ForkSink fork{&a.out, a.count_out + k0, 4u * g_begin, 4u * g_count, a.overflow};
                 BounceCounters cnt{0u, 0u, 0u};
                 alive = bounce_path<STACK>(J, ps, st, cnt, fork);
                 lc.paths += FIRST ? 1u : 0u;
@@ -510,48 +570,29 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
                 lc.emit += cnt.emit;
             }
         }
-        // compaction: a wave reserves its output range with one atomic
-        const uint64_t m = __ballot(alive);
-        const uint32_t off = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        const uint32_t total = (uint32_t)__popcll(m);
-        uint32_t wbase = 0;
-        if (total) {
-            if (lane == 0) wbase = atomicAdd(a.count_out, total);
-            wbase = __shfl(wbase, 0, 64);
-        }
-        if (alive) {
-            uint32_t dst = wbase + off;
-            if (dst < a.capacity) st_store(a.out, dst, ps);
-            else atomicOr(a.overflow, 1u);
-        }
-    }
-    // counters: one atomic per wave when the wave worked on a single job
-    int key = lc.slot < 0 ? 0x7fffffff : lc.slot;
-    int mn = key;
-    for (int dd = 32; dd >= 1; dd >>= 1) mn = min(mn, __shfl_xor(mn, dd, 64));
-    const bool uniform = __ballot(key != mn && key != 0x7fffffff) == 0;
-    if (uniform) {
-        if (mn != 0x7fffffff) {
-            uint32_t p = lc.paths, e = lc.ext, s = lc.shadow, m = lc.emit;
-            for (int dd = 32; dd >= 1; dd >>= 1) {
-                p += __shfl_down(p, dd, 64);
-                e += __shfl_down(e, dd, 64);
-                s += __shfl_down(s, dd, 64);
-                m += __shfl_down(m, dd, 64);
-            }
-            if (lane == 0) {
-                LaneCounters w;
-                w.slot = mn;
-                w.paths = p;
-                w.ext = e;
-                w.shadow = s;
-                w.emit = m;
-                flush_lane(w, a.counters);
+        // compaction into each job's region: one atomic per (wave, job)
+        bool app = alive;
+        while (__ballot(app)) {
+            const uint64_t am = __ballot(app);
+            const int k0 = __builtin_amdgcn_readfirstlane(__shfl(seg, __ffsll((unsigned long long)am) - 1, 64));
+            const bool mine = app && seg == k0;
+            const uint64_t m = __ballot(mine);
+            const uint32_t off = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            const uint32_t total = (uint32_t)__popcll(m);
+            uint32_t wbase = 0;
+            if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) wbase = atomicAdd(a.count_out + k0, total);
+            wbase = __shfl(wbase, __ffsll((unsigned long long)m) - 1, 64);
+            if (mine) {
+                app = false;
+                const uint32_t g_begin = __builtin_amdgcn_readfirstlane(s_segs[k0].begin);
+                const uint32_t g_count = __builtin_amdgcn_readfirstlane(s_segs[k0].count);
+                const uint32_t dst = wbase + off;
+                if (dst < 4u * g_count) st_store(a.out, 4u * g_begin + dst, ps);
+                else atomicOr(a.overflow, 1u);
             }
         }
-    } else if (lc.slot >= 0) {
-        flush_lane(lc, a.counters);
     }
+    if (wslot >= 0) wave_flush(lc, wslot, a.counters);
 }
 
 // Per pixel of each job in the pass: radiance -> XYZ (singleLambdaXYZ,

@@ -1,0 +1,179 @@
+"""Serving benchmark: a seeded stream of render jobs with Poisson arrivals over
+device-resident scenes, run twice on one GPU:
+
+  batched         every job is submitted at its arrival time to one context;
+                  the library admits jobs as slots and memory allow and
+                  renders all admitted jobs' paths in shared wavefront passes
+                  (continuous batching);
+  one-at-a-time   the same stream with one job slot: plain wavefront
+                  rendering, each job alone on the GPU, first come first
+                  served.  A job's render time here is its isolated latency.
+
+Reports, for each run: makespan, job throughput, path throughput, p50/p99 of
+the latency from arrival to result (the library's submit-to-finish clock) and
+of the slowdown (latency / isolated latency).  Every job's FP64 result must
+be identical in both runs (a job's samples do not depend on its pass mates).
+
+    python -m pcr_amd.serving [--jobs 512] [--rate 40] [--seed 2025] ...
+
+The default stream: 512 jobs, 40 jobs/s, scenes drawn uniformly from the
+256x256-class box (cornell-c0), the mirror + glass box (cornell-c1), the
+81,932-triangle displaced icosphere (cornell-c2) and the closed box with a
+30x30 light (closed-c3); square resolution uniform in {128, 256, 512, 720,
+1024}, spp log-uniform in [16, 1024], depth uniform in 2..8, 4 shadow rays,
+spectral with 4 hero wavelengths.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import Context, HostScene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SCENES = ("cornell-c0", "cornell-c1", "cornell-c2", "closed-c3")
+RESOLUTIONS = (128, 256, 512, 720, 1024)
+
+
+def make_stream(n=512, rate=40.0, seed=2025, resolutions=RESOLUTIONS, spp_range=(16, 1024), depth_range=(2, 8),
+                scenes=SCENES):
+    """The job stream: a list of dicts with arrival time t (s), scene index and
+    the job_desc arguments, all drawn from one seeded generator."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    gaps = g.exponential(1.0 / rate, n)
+    t = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    res = g.choice(np.asarray(resolutions), n)
+    spp = np.rint(np.exp(g.uniform(math.log(spp_range[0]), math.log(spp_range[1]), n))).astype(int)
+    depth = g.integers(depth_range[0], depth_range[1] + 1, n)
+    scene = g.integers(0, len(scenes), n)
+    seeds = g.integers(1, 2 ** 63, n, dtype=np.int64)
+    return [dict(t=float(t[i]), scene=int(scene[i]),
+                 desc=dict(width=int(res[i]), height=int(res[i]), depth=int(depth[i]), spp=int(spp[i]), shadow=4,
+                           hero=4, spectral=True, seed=int(seeds[i])))
+            for i in range(n)]
+
+
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
+
+
+def run_stream(ctx, dscenes, stream, keep_sums=False, progress=10.0):
+    """Submits each job at its arrival time (relative to the start), collects
+    every job as it finishes, and returns per-job records (stats in ms) and
+    the makespan (s).  With progress > 0, prints a line every `progress`
+    seconds."""
+    n = len(stream)
+    jobs = [None] * n
+    recs = [None] * n
+    lock = threading.Lock()
+    n_sub = [0]
+
+    def collector():
+        left = set()
+        nxt = 0
+        last = time.perf_counter()
+        done = 0
+        while done < n:
+            with lock:
+                while nxt < n_sub[0]:
+                    left.add(nxt)
+                    nxt += 1
+            finished = [i for i in left if jobs[i].wait(0)]
+            for i in finished:
+                j = jobs[i]
+                _, st = j.result()
+                recs[i] = dict(ms_queue=st.ms_queue, ms_render=st.ms_render, ms_total=st.ms_total,
+                               paths=st.paths, passes=st.passes, sums=j.sums() if keep_sums else None)
+                j.release()
+                left.discard(i)
+                done += 1
+            now = time.perf_counter()
+            if progress and now - last >= progress:
+                print("  %d/%d jobs done, %d submitted, at %.1f s" % (done, n, nxt, now - t0), flush=True)
+                last = now
+            if not finished:
+                time.sleep(0.02)
+
+    th = threading.Thread(target=collector)
+    t0 = time.perf_counter()
+    th.start()
+    for i, spec in enumerate(stream):
+        delay = t0 + spec["t"] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        jobs[i] = dscenes[spec["scene"]].submit(**spec["desc"])
+        with lock:
+            n_sub[0] = i + 1
+    th.join()
+    makespan = time.perf_counter() - t0
+    return recs, makespan
+
+
+def summarize(recs, makespan, isolated=None):
+    lat = [r["ms_total"] for r in recs]
+    out = dict(makespan_s=round(makespan, 3), jobs_per_s=round(len(recs) / makespan, 3),
+               mpaths_s=round(sum(r["paths"] for r in recs) / makespan / 1e6, 2),
+               p50_ms=round(percentile(lat, 50), 2), p99_ms=round(percentile(lat, 99), 2),
+               mean_ms=round(float(np.mean(lat)), 2))
+    if isolated is not None:
+        slow = [r["ms_total"] / max(iso, 1e-3) for r, iso in zip(recs, isolated)]
+        out.update(p50_slowdown=round(percentile(slow, 50), 3), p99_slowdown=round(percentile(slow, 99), 3))
+    return out
+
+
+def benchmark(stream, scene_dir=os.path.join(HERE, "scenes"), device=0, keep_sums=False, progress=10.0, log=print):
+    """Runs the stream batched, then one at a time.  Returns (report, batched
+    records, one-at-a-time records)."""
+    ctx = Context(device)
+    try:
+        dscenes = [ctx.upload(HostScene(os.path.join(scene_dir, s + ".json"))) for s in SCENES]
+        cfg0 = ctx.config()
+        log("batched: %d jobs" % len(stream))
+        rb, mb = run_stream(ctx, dscenes, stream, keep_sums, progress)
+        log("one at a time: %d jobs" % len(stream))
+        ctx.set_config(max_active_jobs=1)
+        ro, mo = run_stream(ctx, dscenes, stream, keep_sums, progress)
+        ctx.set_config(max_active_jobs=cfg0.max_active_jobs)
+        for ds in dscenes:
+            ds.release()
+    finally:
+        ctx.close()
+    isolated = [r["ms_render"] for r in ro]
+    b = summarize(rb, mb, isolated)
+    o = summarize(ro, mo, isolated)
+    iso = dict(p50_ms=round(percentile(isolated, 50), 2), p99_ms=round(percentile(isolated, 99), 2))
+    report = dict(jobs=len(stream), batched=b, one_at_a_time=o, isolated=iso,
+                  throughput_ratio=round(b["jobs_per_s"] / o["jobs_per_s"], 3),
+                  p99_latency_vs_isolated_p99=round(b["p99_ms"] / iso["p99_ms"], 3),
+                  p50_latency_ratio_batched_vs_one_at_a_time=round(b["p50_ms"] / o["p50_ms"], 4))
+    return report, rb, ro
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--rate", type=float, default=40.0, help="mean arrivals per second")
+    ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--spp-max", type=int, default=1024)
+    ap.add_argument("--res", type=str, default=",".join(map(str, RESOLUTIONS)))
+    ap.add_argument("--check", action="store_true", help="also check every job's result is identical in both runs")
+    a = ap.parse_args(argv)
+    stream = make_stream(a.jobs, a.rate, a.seed, tuple(int(x) for x in a.res.split(",")), (16, a.spp_max))
+    report, rb, ro = benchmark(stream, keep_sums=a.check)
+    if a.check:
+        same = all(np.array_equal(x["sums"].view(np.uint64), y["sums"].view(np.uint64)) for x, y in zip(rb, ro))
+        report["alone_equals_mixed"] = same
+    report["stream"] = dict(seed=a.seed, rate=a.rate, spp=[16, a.spp_max], res=a.res, depth=[2, 8], scenes=SCENES,
+                            offered_mpaths=round(sum(s["desc"]["width"] ** 2 * s["desc"]["spp"] for s in stream) / 1e6,
+                                                 1))
+    print(json.dumps(report), flush=True)
+    return 0 if report.get("alone_equals_mixed", True) else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())
