@@ -81,6 +81,11 @@ int pcrhip_destroy(pcrhip_ctx *ctx);
  *              MetalRenderer.mm:1794-1912). */
 enum { PCRHIP_SCHED_GRAPH = 0, PCRHIP_SCHED_PERSISTENT = 1, PCRHIP_SCHED_STAGED = 2 };
 
+/* Admission order among queued jobs of equal priority: fewest remaining paths
+ * first (the default; the GUI batch queue of the reference is FIFO,
+ * Gui/main.cpp:2313-2400), or arrival order. */
+enum { PCRHIP_ADMIT_SHORTEST = 0, PCRHIP_ADMIT_FIFO = 1 };
+
 /* Launch and batching knobs; the counterpart of the reference's threadgroup
  * and samples-per-pass defaults (GpuDefaults.h:15-50, MetalRenderer.mm:3799-3823). */
 typedef struct {
@@ -90,6 +95,7 @@ typedef struct {
     uint64_t memory_budget;      /* bytes of per-job buffers the admission may hold */
     int max_active_jobs;         /* concurrent job slots (<= 64) */
     int schedule;                /* PCRHIP_SCHED_* */
+    int admission;               /* PCRHIP_ADMIT_*: order of queued jobs of equal priority */
 } pcrhip_config;
 int pcrhip_get_config(pcrhip_ctx *ctx, pcrhip_config *out);
 /* Takes effect for passes started after the call. */

@@ -21,6 +21,7 @@ PCRHIP_OK, PCRHIP_EINVAL, PCRHIP_EDEVICE, PCRHIP_ENOMEM = 0, -1, -2, -3
 PCRHIP_ECANCELLED, PCRHIP_ETIMEOUT, PCRHIP_ESTATE = -4, -5, -6
 SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER = 1, 2, 4, 8, 16
 SCHED_GRAPH, SCHED_PERSISTENT, SCHED_STAGED = 0, 1, 2
+ADMIT_SHORTEST, ADMIT_FIFO = 0, 1
 
 _f = C.POINTER(C.c_float)
 _d = C.POINTER(C.c_double)
@@ -47,7 +48,8 @@ class SceneDesc(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("target_pass_paths", C.c_uint64),
-                ("memory_budget", C.c_uint64), ("max_active_jobs", C.c_int), ("schedule", C.c_int)]
+                ("memory_budget", C.c_uint64), ("max_active_jobs", C.c_int), ("schedule", C.c_int),
+                ("admission", C.c_int)]
 
 
 class TuneEntry(C.Structure):

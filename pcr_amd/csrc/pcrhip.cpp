@@ -2,8 +2,9 @@
 // loop (continuous batching of render jobs into shared wavefront queues) and
 // the pass driver that launches the kernels of wavefront.hip.
 //
-// Serving model.  Each admitted job owns a slot of the device job table, an FP64
-// accumulator and a per-pass radiance buffer.  A pass takes every active job's
+// Serving model.  Each admitted job owns a slot of the device job table and an
+// FP64 accumulator (from a pool that keeps freed buffers for reuse: hipFree
+// synchronises the device).  A pass takes every active job's
 // next batch of samples, generates all their paths into one queue, runs the
 // bounce kernels up to the deepest job's depth and resolves every job's pixels.
 // Jobs arriving while a pass runs are admitted at the next pass boundary as
@@ -21,6 +22,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <numbers>
 #include <string>
@@ -102,8 +104,7 @@ struct pcrhip_job {
     int spp_pass = 1;      // samples in the current pass
     int passes = 0;
     double *accum = nullptr;
-    float4 *radiance = nullptr;
-    uint64_t rad_paths = 0;  // capacity of `radiance` in paths
+    uint64_t accum_bytes = 0;  // size class of `accum` in the pool
     uint64_t bytes = 0;
     std::vector<double> sums;
     uint64_t counters[4] = {0, 0, 0, 0};
@@ -125,6 +126,10 @@ struct pcrhip_ctx {
     PassInfo *pass_dev = nullptr;
     QueueView qa{}, qb{};
     uint64_t qcap = 0;
+    float4 *pass_rad = nullptr;  // per-path channel radiance of the current pass
+    uint64_t pass_rad_cap = 0;
+    std::multimap<uint64_t, void *> pool;  // freed accumulators by size class
+    uint64_t pool_bytes = 0;
     DJob *jobs_host = nullptr;  // pinned staging of the job table
     PassInfo *pass_host = nullptr;  // pinned staging of the pass sizes
     // captured pass graphs, keyed by launch shape; dropped when the queues move
@@ -143,6 +148,7 @@ struct pcrhip_ctx {
     pcrhip_job *active[kSlots] = {};
     uint64_t mem_in_use = 0;
     std::string fatal;
+    uint64_t pass_turn = 0;  // rotates passes between traversal-stack classes
     // profiling
     std::atomic<bool> profile{false};
     std::mutex pm;
@@ -223,8 +229,36 @@ void fill_djob(const pcrhip_job *j, DJob &J) {
     J.scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * j->scene->S.fov);
     J.spp_pass = j->spp_pass;
     J.s_begin = j->s_next;
-    J.radiance = j->radiance;
+    J.radiance = nullptr;  // set per pass: a window of the pass radiance buffer
     J.accum = j->accum;
+}
+
+constexpr uint64_t kPoolKeepBytes = 8ull << 30;  // freed accumulators kept for reuse
+
+uint64_t size_class(uint64_t bytes) {
+    uint64_t c = 64 << 10;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+// caller holds c->m
+hipError_t pool_get(pcrhip_ctx *c, uint64_t cls, void **p) {
+    auto it = c->pool.find(cls);
+    if (it != c->pool.end()) {
+        *p = it->second;
+        c->pool.erase(it);
+        c->pool_bytes -= cls;
+        return hipSuccess;
+    }
+    return hipMalloc(p, cls);
+}
+void pool_put(pcrhip_ctx *c, void *p, uint64_t cls) {
+    if (!p) return;
+    if (c->pool_bytes + cls > kPoolKeepBytes) {
+        (void)hipFree(p);
+        return;
+    }
+    c->pool.emplace(cls, p);
+    c->pool_bytes += cls;
 }
 
 void finish_job(pcrhip_ctx *c, pcrhip_job *j, int status, const std::string &err) {
@@ -235,10 +269,8 @@ void finish_job(pcrhip_ctx *c, pcrhip_job *j, int status, const std::string &err
     j->t_done = Clock::now();
     if (j->slot >= 0) c->active[j->slot] = nullptr;
     j->slot = -1;
-    (void)hipFree(j->accum);
-    (void)hipFree(j->radiance);
+    pool_put(c, j->accum, j->accum_bytes);
     j->accum = nullptr;
-    j->radiance = nullptr;
     c->mem_in_use -= j->bytes;
     j->bytes = 0;
     j->scene->jobs--;
@@ -247,9 +279,14 @@ void finish_job(pcrhip_ctx *c, pcrhip_job *j, int status, const std::string &err
 
 // Admits queued jobs into free slots while the memory budget allows.
 void admit(pcrhip_ctx *c) {
-    // highest priority first, FIFO within a priority
-    std::stable_sort(c->pending.begin(), c->pending.end(),
-                     [](const pcrhip_job *a, const pcrhip_job *b) { return a->d.priority > b->d.priority; });
+    // highest priority first; within a priority the fewest paths first (or
+    // arrival order), so small jobs are not held behind large ones
+    const bool sjf = c->cfg.admission == PCRHIP_ADMIT_SHORTEST;
+    std::stable_sort(c->pending.begin(), c->pending.end(), [sjf](const pcrhip_job *a, const pcrhip_job *b) {
+        if (a->d.priority != b->d.priority) return a->d.priority > b->d.priority;
+        if (!sjf) return false;
+        return (uint64_t)a->cw * a->ch * (uint64_t)a->d.spp < (uint64_t)b->cw * b->ch * (uint64_t)b->d.spp;
+    });
     int active = 0;
     for (auto *j : c->active) active += j != nullptr;
     while (!c->pending.empty() && active < std::min(c->cfg.max_active_jobs, kSlots)) {
@@ -260,8 +297,7 @@ void admit(pcrhip_ctx *c) {
             continue;
         }
         uint64_t pix = (uint64_t)j->cw * j->ch;
-        uint64_t spp_pass = std::max<uint64_t>(1, std::min<uint64_t>(j->d.spp, c->cfg.target_pass_paths / pix));
-        uint64_t bytes = pix * 3 * sizeof(double) + pix * spp_pass * sizeof(float4);
+        uint64_t bytes = size_class(pix * 3 * sizeof(double));
         if (active > 0 && c->mem_in_use + bytes > c->cfg.memory_budget) break;  // wait for memory
         if (bytes > c->cfg.memory_budget) {
             c->pending.pop_front();
@@ -272,23 +308,27 @@ void admit(pcrhip_ctx *c) {
         for (int s = 0; s < kSlots; s++)
             if (!c->active[s]) { slot = s; break; }
         if (slot < 0) break;
-        hipError_t e1 = hipMalloc(&j->accum, pix * 3 * sizeof(double));
-        hipError_t e2 = e1 == hipSuccess ? hipMalloc(&j->radiance, pix * spp_pass * sizeof(float4)) : e1;
-        hipError_t e3 = e2 == hipSuccess ? hipMemsetAsync(j->accum, 0, pix * 3 * sizeof(double), c->stream) : e2;
+        hipError_t e1 = pool_get(c, bytes, (void **)&j->accum);
+        hipError_t e2 = e1 == hipSuccess ? hipMemsetAsync(j->accum, 0, pix * 3 * sizeof(double), c->stream) : e1;
         c->pending.pop_front();
         j->bytes = bytes;
+        j->accum_bytes = e1 == hipSuccess ? bytes : 0;
         c->mem_in_use += bytes;
-        if (e3 != hipSuccess) {
-            finish_job(c, j, PCRHIP_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e3));
+        if (e2 != hipSuccess) {
+            finish_job(c, j, PCRHIP_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e2));
             continue;
         }
-        j->rad_paths = pix * spp_pass;
         j->slot = slot;
         j->state = pcrhip_job::RUNNING;
         j->t_admit = Clock::now();
         c->active[slot] = j;
         active++;
     }
+}
+
+// The bounce kernel's traversal-stack variant (8, 16, 32 or 64 entries).
+int stack_class(int stack) {
+    return stack <= 8 ? 8 : stack <= 16 ? 16 : stack <= 32 ? 32 : 64;
 }
 
 // Launch shape of one pass.
@@ -387,15 +427,15 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     int max_depth = 0, stack = 8;
     // Paths per job in this pass: the pass budget water-filled over the jobs
     // (a job that needs less than an equal share takes what it needs and the
-    // rest is shared by the others), at least one sample per pixel, at most
-    // what the job's radiance buffer holds.  Small jobs therefore finish in few
-    // passes while large ones progress at an equal rate.
+    // rest is shared by the others), at least one sample per pixel.  Small
+    // jobs therefore finish in few passes while large ones progress at an
+    // equal rate.
     const size_t nj = jobs.size();
     std::vector<uint64_t> want(nj), alloc(nj, 0);
     for (size_t i = 0; i < nj; i++) {
         const pcrhip_job *j = jobs[i];
         uint64_t pix = (uint64_t)j->cw * j->ch;
-        want[i] = std::min<uint64_t>((uint64_t)(j->d.spp - j->s_next) * pix, j->rad_paths);
+        want[i] = (uint64_t)(j->d.spp - j->s_next) * pix;
     }
     {
         uint64_t left = std::max<uint64_t>(cfg.target_pass_paths, 1);
@@ -422,8 +462,7 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     for (size_t i = 0; i < nj; i++) {
         pcrhip_job *j = jobs[i];
         uint64_t pix = (uint64_t)j->cw * j->ch;
-        int spp_pass = (int)std::clamp<uint64_t>(alloc[i] / pix, 1,
-                                                 std::min<uint64_t>(j->d.spp - j->s_next, j->rad_paths / pix));
+        int spp_pass = (int)std::clamp<uint64_t>(alloc[i] / pix, 1, (uint64_t)(j->d.spp - j->s_next));
         j->spp_pass = spp_pass;
         fill_djob(j, c->jobs_host[j->slot]);
         segs.push_back(GenSeg{j->slot, (uint32_t)n_gen, (uint32_t)(pix * spp_pass), 0u});
@@ -437,6 +476,15 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     if (4 * n_gen >= (1ull << 32)) return fail(PCRHIP_EINVAL, "pass too large");
     int rc = ensure_queues(c, 4 * n_gen);  // a dispersion fork emits up to 4 rays per path
     if (rc) return rc;
+    if (n_gen > c->pass_rad_cap) {
+        (void)hipFree(c->pass_rad);
+        c->pass_rad = nullptr;
+        c->pass_rad_cap = 0;
+        uint64_t cap = std::max<uint64_t>(n_gen, 1u << 20);
+        HIPCHK(hipMalloc(&c->pass_rad, cap * sizeof(float4)));
+        c->pass_rad_cap = cap;
+    }
+    for (size_t i = 0; i < nj; i++) c->jobs_host[jobs[i]->slot].radiance = c->pass_rad + segs[i].begin;
     *c->pass_host = PassInfo{(uint32_t)n_gen, (int32_t)segs.size(), (uint32_t)n_pix, (int32_t)rsegs.size()};
     for (auto *j : jobs)
         HIPCHK(hipMemcpyAsync(c->jobs_dev + j->slot, c->jobs_host + j->slot, sizeof(DJob), hipMemcpyHostToDevice,
@@ -527,6 +575,21 @@ void worker_main(pcrhip_ctx *c) {
                 else run.push_back(j);
             }
         if (run.empty()) continue;
+        // One traversal-stack class per pass: the bounce kernel's LDS stack is
+        // sized for the deepest BVH in the pass, so a deep scene in the mix
+        // would cost every other job occupancy.  Classes take turns.
+        {
+            std::vector<int> classes;
+            for (auto *j : run) classes.push_back(stack_class(j->scene->stack));
+            std::sort(classes.begin(), classes.end());
+            classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
+            if (classes.size() > 1) {
+                const int cls = classes[c->pass_turn++ % classes.size()];
+                run.erase(std::remove_if(run.begin(), run.end(),
+                                         [cls](const pcrhip_job *j) { return stack_class(j->scene->stack) != cls; }),
+                          run.end());
+            }
+        }
         const pcrhip_config cfg = c->cfg;
         lk.unlock();
         int rc = run_pass(c, run, cfg);
@@ -795,6 +858,7 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
     c->cfg.memory_budget = 64ull << 30;
     c->cfg.max_active_jobs = 32;
     c->cfg.schedule = PCRHIP_SCHED_GRAPH;
+    c->cfg.admission = PCRHIP_ADMIT_SHORTEST;
     auto cleanup = [&](int rc) {
         (void)hipStreamDestroy(c->stream);
         (void)hipFree(c->jobs_dev);
@@ -837,6 +901,9 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     drop_graphs(c);
     free_queue(c->qa);
     free_queue(c->qb);
+    (void)hipFree(c->pass_rad);
+    for (auto &kv : c->pool) (void)hipFree(kv.second);
+    c->pool.clear();
     (void)hipFree(c->pass_dev);
     (void)hipHostFree(c->pass_host);
     (void)hipFree(c->jobs_dev);
@@ -863,6 +930,8 @@ int pcrhip_set_config(pcrhip_ctx *c, const pcrhip_config *cfg) {
     if (cfg->target_pass_paths < 1024) return fail(PCRHIP_EINVAL, "target_pass_paths too small");
     if (cfg->max_active_jobs < 1 || cfg->max_active_jobs > kSlots) return fail(PCRHIP_EINVAL, "max_active_jobs out of range");
     if (cfg->schedule < PCRHIP_SCHED_GRAPH || cfg->schedule > PCRHIP_SCHED_STAGED) return fail(PCRHIP_EINVAL, "unknown schedule");
+    if (cfg->admission != PCRHIP_ADMIT_SHORTEST && cfg->admission != PCRHIP_ADMIT_FIFO)
+        return fail(PCRHIP_EINVAL, "unknown admission order");
     std::lock_guard<std::mutex> lk(c->m);
     c->cfg = *cfg;
     return PCRHIP_OK;

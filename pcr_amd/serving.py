@@ -9,6 +9,9 @@ device-resident scenes, run twice on one GPU:
                   rendering, each job alone on the GPU, first come first
                   served.  A job's render time here is its isolated latency.
 
+The batched run admits queued jobs smallest first (or in arrival order with
+--admission fifo).
+
 Reports, for each run: makespan, job throughput, path throughput, p50/p99 of
 the latency from arrival to result (the library's submit-to-finish clock) and
 of the slowdown (latency / isolated latency).  Every job's FP64 result must
@@ -33,7 +36,7 @@ import time
 
 import numpy as np
 
-from . import Context, HostScene
+from . import ADMIT_FIFO, ADMIT_SHORTEST, Context, HostScene
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SCENES = ("cornell-c0", "cornell-c1", "cornell-c2", "closed-c3")
@@ -126,7 +129,8 @@ def summarize(recs, makespan, isolated=None):
     return out
 
 
-def benchmark(stream, scene_dir=os.path.join(HERE, "scenes"), device=0, keep_sums=False, progress=10.0, log=print):
+def benchmark(stream, scene_dir=os.path.join(HERE, "scenes"), device=0, keep_sums=False, progress=10.0, log=print,
+              admission=ADMIT_SHORTEST):
     """Runs the stream batched, then one at a time.  Returns (report, batched
     records, one-at-a-time records)."""
     ctx = Context(device)
@@ -134,11 +138,12 @@ def benchmark(stream, scene_dir=os.path.join(HERE, "scenes"), device=0, keep_sum
         dscenes = [ctx.upload(HostScene(os.path.join(scene_dir, s + ".json"))) for s in SCENES]
         cfg0 = ctx.config()
         log("batched: %d jobs" % len(stream))
+        ctx.set_config(admission=admission)
         rb, mb = run_stream(ctx, dscenes, stream, keep_sums, progress)
         log("one at a time: %d jobs" % len(stream))
-        ctx.set_config(max_active_jobs=1)
+        ctx.set_config(max_active_jobs=1, admission=ADMIT_FIFO)
         ro, mo = run_stream(ctx, dscenes, stream, keep_sums, progress)
-        ctx.set_config(max_active_jobs=cfg0.max_active_jobs)
+        ctx.set_config(max_active_jobs=cfg0.max_active_jobs, admission=cfg0.admission)
         for ds in dscenes:
             ds.release()
     finally:
@@ -159,16 +164,20 @@ def main(argv=None):
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--rate", type=float, default=40.0, help="mean arrivals per second")
     ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--spp-min", type=int, default=16)
     ap.add_argument("--spp-max", type=int, default=1024)
+    ap.add_argument("--admission", choices=("shortest", "fifo"), default="shortest")
     ap.add_argument("--res", type=str, default=",".join(map(str, RESOLUTIONS)))
     ap.add_argument("--check", action="store_true", help="also check every job's result is identical in both runs")
     a = ap.parse_args(argv)
-    stream = make_stream(a.jobs, a.rate, a.seed, tuple(int(x) for x in a.res.split(",")), (16, a.spp_max))
-    report, rb, ro = benchmark(stream, keep_sums=a.check)
+    stream = make_stream(a.jobs, a.rate, a.seed, tuple(int(x) for x in a.res.split(",")), (a.spp_min, a.spp_max))
+    report, rb, ro = benchmark(stream, keep_sums=a.check,
+                               admission=ADMIT_SHORTEST if a.admission == "shortest" else ADMIT_FIFO)
     if a.check:
         same = all(np.array_equal(x["sums"].view(np.uint64), y["sums"].view(np.uint64)) for x, y in zip(rb, ro))
         report["alone_equals_mixed"] = same
-    report["stream"] = dict(seed=a.seed, rate=a.rate, spp=[16, a.spp_max], res=a.res, depth=[2, 8], scenes=SCENES,
+    report["stream"] = dict(seed=a.seed, rate=a.rate, spp=[a.spp_min, a.spp_max], res=a.res, depth=[2, 8],
+                            scenes=SCENES, admission=a.admission,
                             offered_mpaths=round(sum(s["desc"]["width"] ** 2 * s["desc"]["spp"] for s in stream) / 1e6,
                                                  1))
     print(json.dumps(report), flush=True)
