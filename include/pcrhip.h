@@ -46,7 +46,12 @@ enum {
     PCRHIP_MIS = 1u << 1,
     PCRHIP_RUSSIAN = 1u << 2,
     PCRHIP_STRATIFIED = 1u << 3,
-    PCRHIP_JITTER = 1u << 4
+    PCRHIP_JITTER = 1u << 4,
+    /* Learned proposal at diffuse vertices: an MLP trained online on the job's
+     * own paths proposes an 8-lobe vMF mixture, MIS-combined 50/50 with cosine
+     * sampling (no counterpart in the reference, whose diffuse bounce is pure
+     * cosine sampling, Renderer.cpp:526-557). */
+    PCRHIP_GUIDED = 1u << 5
 };
 
 /* Material flag bits (pcrhip_scene_desc.mat_flags), Material.h:14-62 */
@@ -184,6 +189,12 @@ typedef struct {
     uint64_t seed;       /* per-path NumGen streams derive from it */
     int x0, y0, cw, ch;  /* crop window; cw = ch = 0 renders the full frame */
     int priority;        /* higher is admitted first */
+    /* PCRHIP_GUIDED training per pass (0 selects the default): Adam steps
+     * (<= 16, default 16), learning rate (default 5e-3), training samples
+     * recorded (default 65536). */
+    int guide_steps;
+    float guide_lr;
+    int guide_records;
 } pcrhip_job_desc;
 
 /* Queues a job on the context's serving thread (continuous batching: the job
@@ -233,6 +244,26 @@ typedef struct {
  * (GpuDefaults.h:15-50). */
 int pcrhip_autotune(pcrhip_ctx *ctx, pcrhip_scene *scene, const pcrhip_job_desc *desc, uint64_t trial_paths,
                     pcrhip_tune_entry *table, int max_entries, int *n_entries);
+
+/* ------------------------------------------------------ learned proposal -- */
+/* Number of FP32 parameters of the guiding network (W1 [128][64], b1 [64],
+ * W2 [64][64], b2 [64], W3 [64][48], b3 [48]; inputs 102..127 and outputs
+ * 40..47 are padding and must be zero). */
+int pcrhip_guide_param_count(void);
+/* The network's GPU forward pass on n vertices (pos xyz, normal xyz; n*6
+ * floats) with the position box lo xyz, hi xyz: out n*40 floats (per lobe:
+ * mean direction xyz unnormalised, log kappa, weight logit).  Verification
+ * entry of the kernels the guided renderer runs. */
+int pcrhip_guide_forward(pcrhip_ctx *ctx, const float *params, const float *box, const float *verts, int n,
+                         float *out);
+/* The GPU training gradient of n records (o xyz, normal xyz, direction xyz,
+ * weight; n*10 floats): d/dparams of -sum_i weight_i log q(dir_i) / sum_i
+ * weight_i, written to grad (param_count floats); loss receives the loss. */
+int pcrhip_guide_gradient(pcrhip_ctx *ctx, const float *params, const float *box, const float *recs, int n,
+                          float *grad, float *loss);
+/* Mean training loss of each pass of a finished guided job (up to n values);
+ * returns the number of passes through *count. */
+int pcrhip_job_guide_losses(pcrhip_job *job, float *out, int n, int *count);
 
 /* ------------------------------------------------------- measurement ----- */
 /* When enabled, the serving loop brackets every bounce-kernel launch with HIP

@@ -1726,6 +1726,217 @@ int orc_make_icosphere(int subdiv, uint32_t seed, double radius, double amp, con
     if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
     return (int)(v.size() / 9);
 }
+
+// ---------------------------------------------------------------- guiding --
+// CPU restatement of the learned directional proposal of the HIP path
+// (pcr_amd/csrc/guide.hip; the reference renderer has none): input encoding,
+// the 3-layer MLP, the 8-lobe vMF mixture and the gradient of the training
+// loss -sum_i w_i log q(dir_i) / sum_i w_i by backpropagation, in FP64.  With
+// `quantize`, the BF16 roundings the GPU applies (weights, encoded inputs,
+// hidden activations, backward-pass operands) are applied at the same points.
+namespace guide {
+constexpr int IN = 128, IN_USED = 102, HID = 64, LOBES = 8, OUT = 40, OUTP = 48;
+constexpr int W1 = 0, B1 = IN * HID, W2 = B1 + HID, B2 = W2 + HID * HID, W3 = B2 + HID, B3 = W3 + HID * OUTP,
+              P = B3 + OUTP;
+constexpr double LKMIN = -3.0, LKMAX = 7.0;
+
+inline float bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+inline double q(double v, bool quant) { return quant ? (double)bf16((float)v) : v; }
+
+// encoded row (floats, BF16-rounded when quantised) of vertex p, normal n
+void encode(const float *box, const float *p, const float *n, bool quant, double *x) {
+    float s[6];
+    for (int k = 0; k < 3; k++) {
+        float inv = 1.f / (box[3 + k] - box[k]);
+        s[k] = (p[k] - box[k]) * inv;
+        s[3 + k] = (n[k] + 1.f) * 0.5f;
+    }
+    for (int d = 0; d < 6; d++) x[d] = quant ? bf16(s[d]) : s[d];
+    float f = 0.5f;
+    for (int j = 0; j < 8; j++) {
+        for (int d = 0; d < 6; d++) {
+            float sn, cs;
+            fm_sincos_turns(s[d] * f, sn, cs);
+            x[6 + 12 * j + 2 * d] = quant ? bf16(sn) : sn;
+            x[6 + 12 * j + 2 * d + 1] = quant ? bf16(cs) : cs;
+        }
+        f = f * 2.f;
+    }
+    for (int k = IN_USED; k < IN; k++) x[k] = 0.0;
+}
+
+struct Fwd {
+    double x[IN], h1[HID], h2[HID], o[OUTP];
+};
+void forward(const float *w, const float *box, const float *p, const float *n, bool quant, Fwd &F) {
+    encode(box, p, n, quant, F.x);
+    for (int o = 0; o < HID; o++) {
+        double a = 0.0;
+        for (int i = 0; i < IN; i++) a += F.x[i] * q(w[W1 + i * HID + o], quant);
+        a = quant ? (double)((float)a + w[B1 + o]) : a + w[B1 + o];
+        F.h1[o] = q(std::max(0.0, a), quant);
+    }
+    for (int o = 0; o < HID; o++) {
+        double a = 0.0;
+        for (int i = 0; i < HID; i++) a += F.h1[i] * q(w[W2 + i * HID + o], quant);
+        a = quant ? (double)((float)a + w[B2 + o]) : a + w[B2 + o];
+        F.h2[o] = q(std::max(0.0, a), quant);
+    }
+    for (int o = 0; o < OUTP; o++) {
+        double a = 0.0;
+        for (int i = 0; i < HID; i++) a += F.h2[i] * q(w[W3 + i * OUTP + o], quant);
+        F.o[o] = a + w[B3 + o];
+    }
+}
+
+struct LobeD {
+    double mu[3], kappa, e2, len;
+};
+LobeD lobe(const double *o, int k) {
+    LobeD L;
+    double m[3] = {o[5 * k], o[5 * k + 1], o[5 * k + 2]};
+    L.len = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    for (int c = 0; c < 3; c++) L.mu[c] = L.len > 1e-12 ? m[c] / L.len : (c == 2 ? 1.0 : 0.0);
+    L.kappa = std::exp(std::min(std::max(o[5 * k + 3], LKMIN), LKMAX));
+    L.e2 = std::exp(-2.0 * L.kappa);
+    return L;
+}
+double vmf_log_pdf(const double *mu, double kappa, const double *w) {
+    double c = mu[0] * w[0] + mu[1] * w[1] + mu[2] * w[2];
+    return std::log(kappa) - std::log(2.0 * M_PI) - std::log1p(-std::exp(-2.0 * kappa)) + kappa * (c - 1.0);
+}
+void weights(const double *o, double *pi) {
+    double amax = o[4];
+    for (int k = 1; k < LOBES; k++) amax = std::max(amax, o[5 * k + 4]);
+    double s = 0.0;
+    for (int k = 0; k < LOBES; k++) s += (pi[k] = std::exp(o[5 * k + 4] - amax));
+    for (int k = 0; k < LOBES; k++) pi[k] /= s;
+}
+// -scale * dlog q(w)/do; returns -scale * log q(w)
+double out_grad(const double *o, const double *w, double scale, double *d) {
+    double pi[LOBES], t[LOBES];
+    LobeD L[LOBES];
+    weights(o, pi);
+    double tmax = -1e300;
+    for (int k = 0; k < LOBES; k++) {
+        L[k] = lobe(o, k);
+        t[k] = std::log(std::max(pi[k], 1e-300)) + vmf_log_pdf(L[k].mu, L[k].kappa, w);
+        tmax = std::max(tmax, t[k]);
+    }
+    double s = 0.0;
+    for (int k = 0; k < LOBES; k++) s += std::exp(t[k] - tmax);
+    const double logq = tmax + std::log(s);
+    for (int k = 0; k < LOBES; k++) {
+        const double gam = std::exp(t[k] - logq);
+        const double c = L[k].mu[0] * w[0] + L[k].mu[1] * w[1] + L[k].mu[2] * w[2];
+        const double rk = o[5 * k + 3];
+        const bool live = rk > LKMIN && rk < LKMAX;
+        const double kap = L[k].kappa;
+        const double dr = live ? gam * kap * (1.0 / kap - 2.0 * L[k].e2 / (1.0 - L[k].e2) + (c - 1.0)) : 0.0;
+        const double dm = L[k].len > 1e-12 ? gam * kap / L[k].len : 0.0;
+        for (int a = 0; a < 3; a++) d[5 * k + a] = -scale * dm * (w[a] - c * L[k].mu[a]);
+        d[5 * k + 3] = -scale * dr;
+        d[5 * k + 4] = -scale * (gam - pi[k]);
+    }
+    for (int k = OUT; k < OUTP; k++) d[k] = 0.0;
+    return -scale * logq;
+}
+}  // namespace guide
+
+int orc_guide_forward(const float *params, const float *box, const float *verts, int n, int quantize, double *out) {
+    guide::Fwd F;
+    for (int i = 0; i < n; i++) {
+        guide::forward(params, box, verts + 6 * i, verts + 6 * i + 3, quantize != 0, F);
+        for (int k = 0; k < guide::OUT; k++) out[(size_t)i * guide::OUT + k] = F.o[k];
+    }
+    return 0;
+}
+
+int orc_guide_gradient(const float *params, const float *box, const float *recs, int n, int quantize, double *grad,
+                       double *loss) {
+    using namespace guide;
+    const bool qz = quantize != 0;
+    double W = 0.0;
+    for (int i = 0; i < n; i++)
+        if (recs[10 * i + 9] > 0.f) W += recs[10 * i + 9];
+    std::fill(grad, grad + P, 0.0);
+    double L = 0.0;
+    if (!(W > 0.0)) return -1;
+    Fwd F;
+    for (int i = 0; i < n; i++) {
+        const float *r = recs + 10 * i;
+        if (!(r[9] > 0.f)) continue;
+        forward(params, box, r, r + 3, qz, F);
+        const double w[3] = {r[6], r[7], r[8]};
+        double d[OUTP];
+        L += out_grad(F.o, w, r[9] / (qz ? (double)(float)W : W), d);
+        for (int k = 0; k < OUTP; k++) d[k] = q(d[k], qz);
+        double da2[HID], da1[HID];
+        for (int h = 0; h < HID; h++) {
+            double a = 0.0;
+            for (int k = 0; k < OUTP; k++) a += d[k] * q(params[W3 + h * OUTP + k], qz);
+            da2[h] = F.h2[h] > 0.0 ? q(a, qz) : 0.0;
+        }
+        for (int h = 0; h < HID; h++)
+            for (int k = 0; k < OUTP; k++) grad[W3 + h * OUTP + k] += F.h2[h] * d[k];
+        for (int k = 0; k < OUTP; k++) grad[B3 + k] += d[k];
+        for (int h = 0; h < HID; h++) {
+            double a = 0.0;
+            for (int k = 0; k < HID; k++) a += da2[k] * q(params[W2 + h * HID + k], qz);
+            da1[h] = F.h1[h] > 0.0 ? q(a, qz) : 0.0;
+        }
+        for (int h = 0; h < HID; h++)
+            for (int k = 0; k < HID; k++) grad[W2 + h * HID + k] += F.h1[h] * da2[k];
+        for (int k = 0; k < HID; k++) grad[B2 + k] += da2[k];
+        for (int x = 0; x < IN; x++)
+            for (int k = 0; k < HID; k++) grad[W1 + x * HID + k] += F.x[x] * da1[k];
+        for (int k = 0; k < HID; k++) grad[B1 + k] += da1[k];
+    }
+    // padded output columns carry no gradient
+    for (int h = 0; h < HID; h++)
+        for (int k = OUT; k < OUTP; k++) grad[W3 + h * OUTP + k] = 0.0;
+    for (int k = OUT; k < OUTP; k++) grad[B3 + k] = 0.0;
+    if (loss) *loss = L;
+    return 0;
+}
+
+double orc_vmf_pdf(const double mu[3], double kappa, const double w[3]) {
+    return std::exp(guide::vmf_log_pdf(mu, kappa, w));
+}
+// Wood (1994): cos(theta) = 1 + log(u1 + (1 - u1) e^-2k) / k, azimuth 2 pi u2
+void orc_vmf_sample(const double mu[3], double kappa, double u1, double u2, double out[3]) {
+    double c = 1.0 + std::log(u1 + (1.0 - u1) * std::exp(-2.0 * kappa)) / kappa;
+    c = std::min(1.0, std::max(-1.0, c));
+    double s = std::sqrt(std::max(0.0, 1.0 - c * c));
+    double h[3] = {0, 0, 0};
+    if (std::fabs(mu[0]) <= std::fabs(mu[1])) h[0] = 1.0; else h[1] = 1.0;
+    double T[3] = {mu[1] * h[2] - mu[2] * h[1], mu[2] * h[0] - mu[0] * h[2], mu[0] * h[1] - mu[1] * h[0]};
+    double tl = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+    for (double &t : T) t /= tl;
+    double B[3] = {mu[1] * T[2] - mu[2] * T[1], mu[2] * T[0] - mu[0] * T[2], mu[0] * T[1] - mu[1] * T[0]};
+    double ph = 2.0 * M_PI * u2;
+    for (int a = 0; a < 3; a++) out[a] = T[a] * s * std::cos(ph) + B[a] * s * std::sin(ph) + mu[a] * c;
+}
+// density of the learned part (vMF mixture) of network outputs o[40] at w
+double orc_guide_mix_pdf(const double *o, const double w[3]) {
+    double pi[guide::LOBES];
+    guide::weights(o, pi);
+    double p = 0.0;
+    for (int k = 0; k < guide::LOBES; k++) {
+        guide::LobeD L = guide::lobe(o, k);
+        p += pi[k] * std::exp(guide::vmf_log_pdf(L.mu, L.kappa, w));
+    }
+    return p;
+}
+
 void orc_fm_sincos_turns(float t, float out[2]) { fm_sincos_turns(t, out[0], out[1]); }
 float orc_fm_exp(float x) { return fm_exp(x); }
 void orc_fm_cmf(float l, float out[3]) {

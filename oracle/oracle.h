@@ -117,6 +117,16 @@ int orc_spd_resample(const float *lam, const float *val, int n, float out61[61])
 int orc_make_icosphere(int subdiv, uint32_t seed, double radius, double amp, const double center[3],
                        float *out);
 /* IEEE-only math used by the non-branching estimator */
+/* Learned proposal of the HIP path (restated in FP64; `quantize` applies the
+ * GPU's BF16 roundings): forward pass on n vertices (pos, normal; n*6) ->
+ * n*40 outputs; gradient of -sum w_i log q(dir_i) / sum w_i over n records
+ * (o, normal, dir, weight; n*10) -> grad[param_count], loss. */
+int orc_guide_forward(const float *params, const float *box, const float *verts, int n, int quantize, double *out);
+int orc_guide_gradient(const float *params, const float *box, const float *recs, int n, int quantize, double *grad,
+                       double *loss);
+double orc_vmf_pdf(const double mu[3], double kappa, const double w[3]);
+void orc_vmf_sample(const double mu[3], double kappa, double u1, double u2, double out[3]);
+double orc_guide_mix_pdf(const double *o40, const double w[3]);
 void orc_fm_sincos_turns(float t, float out[2]);
 float orc_fm_exp(float x);
 void orc_fm_cmf(float lambda, float out[3]);

@@ -19,7 +19,7 @@ HOST_LIB = os.path.join(BUILD, "libpcrhost.so")
 
 PCRHIP_OK, PCRHIP_EINVAL, PCRHIP_EDEVICE, PCRHIP_ENOMEM = 0, -1, -2, -3
 PCRHIP_ECANCELLED, PCRHIP_ETIMEOUT, PCRHIP_ESTATE = -4, -5, -6
-SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER = 1, 2, 4, 8, 16
+SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER, GUIDED = 1, 2, 4, 8, 16, 32
 SCHED_GRAPH, SCHED_PERSISTENT, SCHED_STAGED = 0, 1, 2
 ADMIT_SHORTEST, ADMIT_FIFO = 0, 1
 
@@ -65,7 +65,8 @@ class SceneInfo(C.Structure):
 class JobDesc(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("depth", C.c_int), ("spp", C.c_int),
                 ("shadow", C.c_int), ("hero", C.c_int), ("flags", C.c_uint32), ("seed", C.c_uint64),
-                ("x0", C.c_int), ("y0", C.c_int), ("cw", C.c_int), ("ch", C.c_int), ("priority", C.c_int)]
+                ("x0", C.c_int), ("y0", C.c_int), ("cw", C.c_int), ("ch", C.c_int), ("priority", C.c_int),
+                ("guide_steps", C.c_int), ("guide_lr", C.c_float), ("guide_records", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -102,6 +103,10 @@ HIP_API = {
     "pcrhip_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), _f, C.c_size_t, C.POINTER(Stats)]),
     "pcrhip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(JobDesc), C.c_uint64, C.POINTER(TuneEntry),
                                   C.c_int, C.POINTER(C.c_int)]),
+    "pcrhip_guide_param_count": (C.c_int, []),
+    "pcrhip_guide_forward": (C.c_int, [C.c_void_p, _f, _f, _f, C.c_int, _f]),
+    "pcrhip_guide_gradient": (C.c_int, [C.c_void_p, _f, _f, _f, C.c_int, _f, _f]),
+    "pcrhip_job_guide_losses": (C.c_int, [C.c_void_p, _f, C.c_int, C.POINTER(C.c_int)]),
     "pcrhip_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "pcrhip_profile_get": (C.c_int, [C.c_void_p, C.POINTER(Profile), C.c_int]),
 }
@@ -232,6 +237,26 @@ class Context:
         check(hip().pcrhip_scene_upload(self.h, C.byref(host_scene.desc), C.byref(s)))
         return DeviceScene(self, s)
 
+    def guide_forward(self, params, box, verts):
+        """The guiding network's GPU forward pass: verts (n, 6) -> (n, 40)."""
+        params = np.ascontiguousarray(params, np.float32)
+        box = np.ascontiguousarray(box, np.float32)
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 6)
+        out = np.zeros((len(verts), 40), np.float32)
+        check(hip().pcrhip_guide_forward(self.h, fp(params), fp(box), fp(verts), len(verts), fp(out)))
+        return out
+
+    def guide_gradient(self, params, box, recs):
+        """The GPU training gradient of records (n, 10): returns (grad, loss)."""
+        params = np.ascontiguousarray(params, np.float32)
+        box = np.ascontiguousarray(box, np.float32)
+        recs = np.ascontiguousarray(recs, np.float32).reshape(-1, 10)
+        grad = np.zeros(hip().pcrhip_guide_param_count(), np.float32)
+        loss = C.c_float()
+        check(hip().pcrhip_guide_gradient(self.h, fp(params), fp(box), fp(recs), len(recs), fp(grad),
+                                          C.byref(loss)))
+        return grad, loss.value
+
     def profile(self, enable=None, reset=False):
         if enable is not None:
             check(hip().pcrhip_profile_enable(self.h, int(enable)))
@@ -291,11 +316,13 @@ class DeviceScene:
 
 
 def job_desc(width, height, depth, spp, shadow, seed, spectral=True, hero=4, mis=False, russian=False,
-             stratified=False, jitter=False, crop=None, priority=0):
+             stratified=False, jitter=False, crop=None, priority=0, guided=False, guide_steps=0, guide_lr=0.0,
+             guide_records=0):
     flags = (SPECTRAL if spectral else 0) | (MIS if mis else 0) | (RUSSIAN if russian else 0) | \
-            (STRATIFIED if stratified else 0) | (JITTER if jitter else 0)
+            (STRATIFIED if stratified else 0) | (JITTER if jitter else 0) | (GUIDED if guided else 0)
     x0, y0, cw, ch = crop if crop else (0, 0, 0, 0)
-    return JobDesc(width, height, depth, spp, shadow, hero, flags, seed, x0, y0, cw, ch, priority)
+    return JobDesc(width, height, depth, spp, shadow, hero, flags, seed, x0, y0, cw, ch, priority, guide_steps,
+                   guide_lr, guide_records)
 
 
 class Job:
@@ -327,6 +354,14 @@ class Job:
         st = Stats()
         check(hip().pcrhip_job_result(self.h, fp(out), out.size, C.byref(st)))
         return out, st
+
+    def guide_losses(self):
+        """Mean training loss of each pass (guided jobs; after wait())."""
+        n = C.c_int()
+        check(hip().pcrhip_job_guide_losses(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value), np.float32)
+        check(hip().pcrhip_job_guide_losses(self.h, fp(out), n.value, C.byref(n)))
+        return out[:n.value]
 
     def sums(self):
         out = np.zeros((self.ch, self.cw, 3), np.float64)

@@ -29,6 +29,7 @@
 #include <thread>
 #include <vector>
 
+#include "guide.h"
 #include "pcr_kernels.h"
 
 using namespace pcr;
@@ -88,6 +89,7 @@ struct pcrhip_scene {
     int n_inner = 0, n_leaves = 0;
     uint64_t bytes = 0;
     std::atomic<int> jobs{0};
+    float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // bounds of all geometry
 };
 
 struct pcrhip_job {
@@ -105,6 +107,12 @@ struct pcrhip_job {
     int passes = 0;
     double *accum = nullptr;
     uint64_t accum_bytes = 0;  // size class of `accum` in the pool
+    // learned proposal (PCRHIP_GUIDED)
+    bool guided = false;
+    GuideNet net{};
+    void *net_mem = nullptr;
+    int adam_t = 0;
+    std::vector<float> guide_loss;  // mean training loss per pass
     uint64_t bytes = 0;
     std::vector<double> sums;
     uint64_t counters[4] = {0, 0, 0, 0};
@@ -128,6 +136,8 @@ struct pcrhip_ctx {
     uint64_t qcap = 0;
     float4 *pass_rad = nullptr;  // per-path channel radiance of the current pass
     uint64_t pass_rad_cap = 0;
+    GuideRecord *grec = nullptr;  // training records of a guided pass
+    uint32_t grec_cap = 0;
     std::multimap<uint64_t, void *> pool;  // freed accumulators by size class
     uint64_t pool_bytes = 0;
     DJob *jobs_host = nullptr;  // pinned staging of the job table
@@ -205,6 +215,8 @@ int validate_job(const pcrhip_scene *s, const pcrhip_job_desc *d) {
     if ((uint64_t)d->width * d->height * (uint64_t)d->spp >= (1ull << 62))
         return fail(PCRHIP_EINVAL, "too many samples");
     if (s->S.n_lights < 1) return fail(PCRHIP_EINVAL, "scene has no light");
+    if (d->guide_steps < 0 || d->guide_steps > kGMaxSteps) return fail(PCRHIP_EINVAL, "guide_steps must be in 0..16");
+    if (!(d->guide_lr >= 0.f) || d->guide_records < 0) return fail(PCRHIP_EINVAL, "bad guide_lr or guide_records");
     return PCRHIP_OK;
 }
 
@@ -271,10 +283,68 @@ void finish_job(pcrhip_ctx *c, pcrhip_job *j, int status, const std::string &err
     j->slot = -1;
     pool_put(c, j->accum, j->accum_bytes);
     j->accum = nullptr;
+    if (j->net_mem) (void)hipFree(j->net_mem);
+    j->net_mem = nullptr;
     c->mem_in_use -= j->bytes;
     j->bytes = 0;
     j->scene->jobs--;
     c->done_cv.notify_all();
+}
+
+// Initial guiding weights, seeded by the job: Glorot-uniform layers, the output
+// layer scaled by 0.1 so the first mixture is a set of broad lobes (log kappa
+// near 0, equal weights) about directions that vary smoothly over the scene.
+std::vector<float> guide_init(uint64_t seed) {
+    std::vector<float> w(kGParams, 0.f);
+    uint64_t st = seed ^ 0x2545F4914F6CDD1Dull;
+    auto uni = [&]() {  // splitmix64 -> [-1, 1)
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (float)((double)(z >> 40) / 16777216.0) * 2.f - 1.f;
+    };
+    const float a1 = std::sqrt(6.f / (kGInUsed + kGHid)), a2 = std::sqrt(6.f / (2 * kGHid)),
+                a3 = 0.1f * std::sqrt(6.f / (kGHid + kGOut));
+    for (int i = 0; i < kGInUsed; i++)
+        for (int o = 0; o < kGHid; o++) w[kGOffW1 + i * kGHid + o] = a1 * uni();
+    for (int i = 0; i < kGHid * kGHid; i++) w[kGOffW2 + i] = a2 * uni();
+    for (int i = 0; i < kGHid; i++)
+        for (int o = 0; o < kGOut; o++) w[kGOffW3 + i * kGOutPad + o] = a3 * uni();
+    for (int k = 0; k < kGLobes; k++)
+        for (int c = 0; c < 3; c++) w[kGOffB3 + 5 * k + c] = uni();
+    return w;
+}
+
+// One allocation holds the job's network state; caller holds c->m.
+hipError_t guide_alloc(pcrhip_ctx *c, pcrhip_job *j) {
+    const size_t P = kGParams, nb = 2 * (size_t)(kGW1 + kGW2 + kGW3);
+    const size_t bytes = 4 * P * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 + nb * sizeof(uint16_t);
+    char *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess) return e;
+    j->net_mem = p;
+    GuideNet &g = j->net;
+    g.w = (float *)p;
+    g.grad = g.w + P;
+    g.m = g.grad + P;
+    g.v = g.m + P;
+    g.wsum = g.v + P;
+    g.n_rec = (uint32_t *)(g.wsum + 2 * kGMaxSteps);
+    g.step = (int32_t *)(g.n_rec + 1);
+    g.wb = (uint16_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+    for (int k = 0; k < 3; k++) {
+        g.lo[k] = j->scene->box_lo[k];
+        g.inv[k] = 1.f / (j->scene->box_hi[k] - j->scene->box_lo[k]);
+    }
+    std::vector<float> w0 = guide_init(j->d.seed);
+    if ((e = hipMemsetAsync(p, 0, bytes, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(g.w, w0.data(), P * sizeof(float), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return e;
+    if ((e = pcr_launch_guide_sync(&g, c->stream)) != hipSuccess) return e;
+    j->guided = true;
+    return hipStreamSynchronize(c->stream);
 }
 
 // Admits queued jobs into free slots while the memory budget allows.
@@ -318,6 +388,10 @@ void admit(pcrhip_ctx *c) {
             finish_job(c, j, PCRHIP_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e2));
             continue;
         }
+        if ((j->d.flags & PCRHIP_GUIDED) && (e2 = guide_alloc(c, j)) != hipSuccess) {
+            finish_job(c, j, PCRHIP_ENOMEM, std::string("guide network allocation failed: ") + hipGetErrorString(e2));
+            continue;
+        }
         j->slot = slot;
         j->state = pcrhip_job::RUNNING;
         j->t_admit = Clock::now();
@@ -329,6 +403,12 @@ void admit(pcrhip_ctx *c) {
 // The bounce kernel's traversal-stack variant (8, 16, 32 or 64 entries).
 int stack_class(int stack) {
     return stack <= 8 ? 8 : stack <= 16 ? 16 : stack <= 32 ? 32 : 64;
+}
+
+// Jobs that may share a pass: same stack variant; a guided job trains its own
+// network on its pass's paths, so it runs alone.
+int pass_class(const pcrhip_job *j) {
+    return j->guided ? 1000 + j->slot : stack_class(j->scene->stack);
 }
 
 // Launch shape of one pass.
@@ -393,6 +473,40 @@ int make_events(std::vector<hipEvent_t> &ev, int n) {
     return PCRHIP_OK;
 }
 
+// A pass of one guided job: after each bounce the guiding kernel draws the
+// pending directions (and writes training records); after the resolve the
+// records get their weights and the network takes S Adam steps.
+int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hipEvent_t> &ev, uint32_t n_gen,
+                        const std::vector<GenSeg> &segs, pcrhip_job *j, uint32_t rec_cap, int S) {
+    GuideNet &g = j->net;
+    HIPCHK(pcr_launch_reset(c->ctrs, kCtrStride * sh.nb + 1, c->stream));
+    HIPCHK(pcr_launch_reset((uint32_t *)c->slot_counters, kSlots * 4 * 2, c->stream));
+    HIPCHK(pcr_launch_reset((uint32_t *)g.wsum, 2 * kGMaxSteps, c->stream));
+    HIPCHK(pcr_launch_reset(g.n_rec, 1, c->stream));
+    // record about rec_cap of the pass's guided vertices (2 per path assumed)
+    const double frac = std::min(1.0, (double)rec_cap / (2.0 * std::max<uint32_t>(n_gen, 1)));
+    const uint32_t keep = (uint32_t)std::min(16777216.0, std::ceil(frac * 16777216.0));
+    for (int b = 0; b < sh.nb; b++) {
+        BounceArgs a = bounce_args(c, sh, b);
+        if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
+        HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, sh.grid, sh.block, c->stream));
+        if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
+        GuideArgs ga{g, a.out, a.count_out, 4u * segs[0].count, c->grec, rec_cap, keep};
+        HIPCHK(pcr_launch_guide(&ga, c->n_cu * 2, c->stream));
+    }
+    ResolveArgs ra{c->jobs_dev, c->rsegs_dev, c->pass_dev};
+    if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb], c->stream));
+    HIPCHK(pcr_launch_resolve(&ra, sh.rgrid, c->stream));
+    if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb + 1], c->stream));
+    HIPCHK(pcr_launch_guide_prep(&g, c->grec, rec_cap, c->pass_rad + segs[0].begin, S, c->stream));
+    const GuideTrain h{j->d.guide_lr > 0.f ? j->d.guide_lr : 5e-3f, 0.9f, 0.99f, 1e-8f};
+    for (int q = 0; q < S; q++) {
+        HIPCHK(pcr_launch_guide_grad(&g, c->grec, rec_cap, q, S, c->n_cu, c->stream));
+        HIPCHK(pcr_launch_guide_adam(&g, &h, ++j->adam_t, c->stream));
+    }
+    return PCRHIP_OK;
+}
+
 // The captured graph of a pass shape, built on first use.
 int pass_graph(pcrhip_ctx *c, const PassShape &sh, pcrhip_ctx::PassGraph **out) {
     for (auto &g : c->graphs)
@@ -437,8 +551,15 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         uint64_t pix = (uint64_t)j->cw * j->ch;
         want[i] = (uint64_t)(j->d.spp - j->s_next) * pix;
     }
+    const bool guided = nj == 1 && jobs[0]->guided;
     {
         uint64_t left = std::max<uint64_t>(cfg.target_pass_paths, 1);
+        if (guided) {
+            // online training: the first passes are small (1, 2, 4, ... samples
+            // per pixel) so the network learns early in the job
+            const uint64_t pix = (uint64_t)jobs[0]->cw * jobs[0]->ch;
+            left = std::min<uint64_t>(left, pix << std::min(jobs[0]->passes, 20));
+        }
         std::vector<size_t> open(nj);
         for (size_t i = 0; i < nj; i++) open[i] = i;
         while (!open.empty()) {
@@ -485,6 +606,14 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         c->pass_rad_cap = cap;
     }
     for (size_t i = 0; i < nj; i++) c->jobs_host[jobs[i]->slot].radiance = c->pass_rad + segs[i].begin;
+    const uint32_t rec_cap = guided ? (uint32_t)(jobs[0]->d.guide_records > 0 ? jobs[0]->d.guide_records : 65536) : 0;
+    if (rec_cap > c->grec_cap) {
+        (void)hipFree(c->grec);
+        c->grec = nullptr;
+        c->grec_cap = 0;
+        HIPCHK(hipMalloc(&c->grec, (size_t)rec_cap * sizeof(GuideRecord)));
+        c->grec_cap = rec_cap;
+    }
     *c->pass_host = PassInfo{(uint32_t)n_gen, (int32_t)segs.size(), (uint32_t)n_pix, (int32_t)rsegs.size()};
     for (auto *j : jobs)
         HIPCHK(hipMemcpyAsync(c->jobs_dev + j->slot, c->jobs_host + j->slot, sizeof(DJob), hipMemcpyHostToDevice,
@@ -501,7 +630,11 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     // launches (same kernels, same grid).
     const bool prof = c->profile.load();
     std::vector<hipEvent_t> ev;
-    if (cfg.schedule == PCRHIP_SCHED_GRAPH && !prof) {
+    const int S = guided ? (jobs[0]->d.guide_steps > 0 ? jobs[0]->d.guide_steps : 16) : 0;
+    if (guided) {
+        if (prof && (rc = make_events(ev, 2 * nb + 2))) return rc;
+        if ((rc = enqueue_guided_pass(c, sh, ev, (uint32_t)n_gen, segs, jobs[0], rec_cap, S))) return rc;
+    } else if (cfg.schedule == PCRHIP_SCHED_GRAPH && !prof) {
         pcrhip_ctx::PassGraph *g = nullptr;
         if ((rc = pass_graph(c, sh, &g))) return rc;
         HIPCHK(hipGraphLaunch(g->exec, c->stream));
@@ -514,8 +647,21 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     std::vector<unsigned long long> sc(kSlots * 4);
     HIPCHK(hipMemcpyAsync(sc.data(), c->slot_counters, sc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c->stream));
+    float gl[2 * kGMaxSteps];
+    if (guided)
+        HIPCHK(hipMemcpyAsync(gl, jobs[0]->net.wsum, 2 * S * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ctr[kCtrStride * nb]) return fail(PCRHIP_EDEVICE, "ray queue overflow");
+    if (guided) {
+        float l = 0.f;
+        int k = 0;
+        for (int q = 0; q < S; q++)
+            if (gl[2 * q] > 0.f) {
+                l += gl[2 * q + 1];
+                k++;
+            }
+        jobs[0]->guide_loss.push_back(k ? l / k : 0.f);
+    }
     if (prof) {
         std::lock_guard<std::mutex> lk(c->pm);
         uint64_t rays_in = n_gen;  // generating bounce reads no queue
@@ -577,16 +723,17 @@ void worker_main(pcrhip_ctx *c) {
         if (run.empty()) continue;
         // One traversal-stack class per pass: the bounce kernel's LDS stack is
         // sized for the deepest BVH in the pass, so a deep scene in the mix
-        // would cost every other job occupancy.  Classes take turns.
+        // would cost every other job occupancy.  Classes take turns; a guided
+        // job is a class of its own.
         {
             std::vector<int> classes;
-            for (auto *j : run) classes.push_back(stack_class(j->scene->stack));
+            for (auto *j : run) classes.push_back(pass_class(j));
             std::sort(classes.begin(), classes.end());
             classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
             if (classes.size() > 1) {
                 const int cls = classes[c->pass_turn++ % classes.size()];
                 run.erase(std::remove_if(run.begin(), run.end(),
-                                         [cls](const pcrhip_job *j) { return stack_class(j->scene->stack) != cls; }),
+                                         [cls](const pcrhip_job *j) { return pass_class(j) != cls; }),
                           run.end());
             }
         }
@@ -824,6 +971,34 @@ int build_scene(pcrhip_scene *s, const pcrhip_scene_desc *d) {
     S.total_area = total;
     S.cam = make_float3(d->cam_pos[0], d->cam_pos[1], d->cam_pos[2]);
     S.fov = d->cam_fov;
+    // bounds of all geometry (input normalisation of the guiding network)
+    {
+        float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+        auto grow = [&](float x, float y, float z) {
+            const float v[3] = {x, y, z};
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], v[k]);
+                hi[k] = std::max(hi[k], v[k]);
+            }
+        };
+        for (int i = 0; i < d->n_spheres; i++) {
+            const float *q = d->spheres + 4 * i;
+            grow(q[0] - q[3], q[1] - q[3], q[2] - q[3]);
+            grow(q[0] + q[3], q[1] + q[3], q[2] + q[3]);
+        }
+        for (int i = 0; i < d->n_planes; i++) {
+            const float *q = d->planes + 9 * i;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    grow(q[0] + a * q[3] + b * q[6], q[1] + a * q[4] + b * q[7], q[2] + a * q[5] + b * q[8]);
+        }
+        for (int i = 0; i < 3 * d->n_triangles; i++) grow(d->triangles[3 * i], d->triangles[3 * i + 1], d->triangles[3 * i + 2]);
+        for (int k = 0; k < 3; k++) {
+            if (!(lo[k] <= hi[k])) lo[k] = 0.f, hi[k] = 1.f;
+            s->box_lo[k] = lo[k];
+            s->box_hi[k] = std::max(hi[k], lo[k] + 1e-3f);
+        }
+    }
     return PCRHIP_OK;
 }
 
@@ -902,6 +1077,7 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     free_queue(c->qa);
     free_queue(c->qb);
     (void)hipFree(c->pass_rad);
+    (void)hipFree(c->grec);
     for (auto &kv : c->pool) (void)hipFree(kv.second);
     c->pool.clear();
     (void)hipFree(c->pass_dev);
@@ -1226,6 +1402,122 @@ int pcrhip_autotune(pcrhip_ctx *c, pcrhip_scene *s, const pcrhip_job_desc *d, ui
     if (n_entries) *n_entries = (int)log.size();
     if (rc) g_err = err;
     return rc;
+}
+
+int pcrhip_guide_param_count(void) { return kGParams; }
+
+// A temporary network with the given parameters on its own stream, for the
+// verification entries.
+struct TempNet {
+    hipStream_t st = nullptr;
+    void *mem = nullptr;
+    GuideNet g{};
+    ~TempNet() {
+        if (st) (void)hipStreamSynchronize(st);
+        (void)hipFree(mem);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    int init(pcrhip_ctx *c, const float *params, const float *box) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        const size_t P = kGParams, nb = 2 * (size_t)(kGW1 + kGW2 + kGW3);
+        const size_t bytes = 4 * P * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 + nb * sizeof(uint16_t);
+        HIPCHK(hipMalloc(&mem, bytes));
+        HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
+        g.w = (float *)mem;
+        g.grad = g.w + P;
+        g.m = g.grad + P;
+        g.v = g.m + P;
+        g.wsum = g.v + P;
+        g.n_rec = (uint32_t *)(g.wsum + 2 * kGMaxSteps);
+        g.step = (int32_t *)(g.n_rec + 1);
+        g.wb = (uint16_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+        for (int k = 0; k < 3; k++) {
+            g.lo[k] = box[k];
+            g.inv[k] = 1.f / (box[3 + k] - box[k]);
+        }
+        HIPCHK(hipMemcpyAsync(g.w, params, P * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHK(pcr_launch_guide_sync(&g, st));
+        return PCRHIP_OK;
+    }
+};
+
+int pcrhip_guide_forward(pcrhip_ctx *c, const float *params, const float *box, const float *verts, int n,
+                         float *out) {
+    if (!c || !params || !box || (n > 0 && (!verts || !out)) || n < 0) return fail(PCRHIP_EINVAL, "bad argument");
+    if (n == 0) return PCRHIP_OK;
+    TempNet t;
+    int rc = t.init(c, params, box);
+    if (rc) return rc;
+    float *dv = nullptr, *dout = nullptr;
+    auto done = [&](int r) {
+        (void)hipStreamSynchronize(t.st);
+        (void)hipFree(dv);
+        (void)hipFree(dout);
+        return r;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&dv, (size_t)n * 6 * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&dout, (size_t)n * kGOut * sizeof(float))) != hipSuccess ||
+        (e = hipMemcpyAsync(dv, verts, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
+        (e = pcr_launch_guide_forward(&t.g, dv, (uint32_t)n, dout, c->n_cu * 2, t.st)) != hipSuccess ||
+        (e = hipMemcpyAsync(out, dout, (size_t)n * kGOut * sizeof(float), hipMemcpyDeviceToHost, t.st)) != hipSuccess ||
+        (e = hipStreamSynchronize(t.st)) != hipSuccess)
+        return done(fail(PCRHIP_EDEVICE, std::string("guide forward: ") + hipGetErrorString(e)));
+    return done(PCRHIP_OK);
+}
+
+int pcrhip_guide_gradient(pcrhip_ctx *c, const float *params, const float *box, const float *recs, int n,
+                          float *grad, float *loss) {
+    if (!c || !params || !box || !grad || n < 1 || !recs) return fail(PCRHIP_EINVAL, "bad argument");
+    TempNet t;
+    int rc = t.init(c, params, box);
+    if (rc) return rc;
+    std::vector<GuideRecord> hr((size_t)n);
+    double W = 0.0;
+    for (int i = 0; i < n; i++) {
+        GuideRecord &R = hr[i];
+        std::memset(&R, 0, sizeof(R));
+        const float *q = recs + 10 * (size_t)i;
+        for (int k = 0; k < 3; k++) {
+            R.o[k] = q[k];
+            R.n[k] = q[3 + k];
+            R.w[k] = q[6 + k];
+        }
+        R.weight = q[9];
+        if (R.weight > 0.f) W += R.weight;
+    }
+    if (!(W > 0.0)) return fail(PCRHIP_EINVAL, "records need a positive weight");
+    GuideRecord *dr = nullptr;
+    auto done = [&](int r) {
+        (void)hipStreamSynchronize(t.st);
+        (void)hipFree(dr);
+        return r;
+    };
+    const float wsum[2] = {(float)W, 0.f};
+    const uint32_t nr = (uint32_t)n;
+    float outw[2];
+    hipError_t e;
+    if ((e = hipMalloc(&dr, (size_t)n * sizeof(GuideRecord))) != hipSuccess ||
+        (e = hipMemcpyAsync(dr, hr.data(), (size_t)n * sizeof(GuideRecord), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t.g.wsum, wsum, sizeof(wsum), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t.g.n_rec, &nr, sizeof(nr), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
+        (e = pcr_launch_guide_grad(&t.g, dr, nr, 0, 1, c->n_cu, t.st)) != hipSuccess ||
+        (e = hipMemcpyAsync(grad, t.g.grad, kGParams * sizeof(float), hipMemcpyDeviceToHost, t.st)) != hipSuccess ||
+        (e = hipMemcpyAsync(outw, t.g.wsum, sizeof(outw), hipMemcpyDeviceToHost, t.st)) != hipSuccess ||
+        (e = hipStreamSynchronize(t.st)) != hipSuccess)
+        return done(fail(PCRHIP_EDEVICE, std::string("guide gradient: ") + hipGetErrorString(e)));
+    if (loss) *loss = outw[1];
+    return done(PCRHIP_OK);
+}
+
+int pcrhip_job_guide_losses(pcrhip_job *j, float *out, int n, int *count) {
+    if (!j) return fail(PCRHIP_EINVAL, "null job");
+    std::lock_guard<std::mutex> lk(j->ctx->m);
+    if (j->state != pcrhip_job::DONE) return fail(PCRHIP_ESTATE, "job not finished");
+    for (int i = 0; i < n && i < (int)j->guide_loss.size(); i++) out[i] = j->guide_loss[i];
+    if (count) *count = (int)j->guide_loss.size();
+    return PCRHIP_OK;
 }
 
 int pcrhip_profile_enable(pcrhip_ctx *c, int enable) {

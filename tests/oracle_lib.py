@@ -89,6 +89,11 @@ def lib():
         "orc_bvh_build": (C.c_int, [_f, C.c_int, _i32, _i32]),
         "orc_spd_resample": (C.c_int, [_f, _f, C.c_int, _f]),
         "orc_make_icosphere": (C.c_int, [C.c_int, C.c_uint32, C.c_double, C.c_double, _d, _f]),
+        "orc_guide_forward": (C.c_int, [_f, _f, _f, C.c_int, C.c_int, _d]),
+        "orc_guide_gradient": (C.c_int, [_f, _f, _f, C.c_int, C.c_int, _d, _d]),
+        "orc_vmf_pdf": (C.c_double, [_d, C.c_double, _d]),
+        "orc_vmf_sample": (None, [_d, C.c_double, C.c_double, C.c_double, _d]),
+        "orc_guide_mix_pdf": (C.c_double, [_d, _d]),
         "orc_fm_sincos_turns": (None, [C.c_float, _f]),
         "orc_fm_exp": (C.c_float, [C.c_float]),
         "orc_fm_cmf": (None, [C.c_float, _f]),
@@ -225,3 +230,66 @@ def spread_seed(i):
     XSH-RR ignores those bits), so statistical tests spread their seeds over
     all 64 bits."""
     return ((i + 1) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+
+
+GUIDE_PARAMS = 128 * 64 + 64 + 64 * 64 + 64 + 64 * 48 + 48
+
+
+def guide_forward(params, box, verts, quantize=True):
+    params, box = f32(params), f32(box)
+    verts = f32(verts).reshape(-1, 6)
+    out = np.zeros((len(verts), 40), np.float64)
+    lib().orc_guide_forward(fp(params), fp(box), fp(verts), len(verts), int(quantize), out.ctypes.data_as(_d))
+    return out
+
+
+def guide_gradient(params, box, recs, quantize=True):
+    params, box = f32(params), f32(box)
+    recs = f32(recs).reshape(-1, 10)
+    grad = np.zeros(GUIDE_PARAMS, np.float64)
+    loss = C.c_double()
+    rc = lib().orc_guide_gradient(fp(params), fp(box), fp(recs), len(recs), int(quantize), grad.ctypes.data_as(_d),
+                                  C.byref(loss))
+    assert rc == 0
+    return grad, loss.value
+
+
+def vmf_pdf(mu, kappa, w):
+    mu = np.ascontiguousarray(mu, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    return lib().orc_vmf_pdf(mu.ctypes.data_as(_d), float(kappa), w.ctypes.data_as(_d))
+
+
+def vmf_sample(mu, kappa, u1, u2):
+    mu = np.ascontiguousarray(mu, np.float64)
+    out = np.zeros(3, np.float64)
+    lib().orc_vmf_sample(mu.ctypes.data_as(_d), float(kappa), float(u1), float(u2), out.ctypes.data_as(_d))
+    return out
+
+
+def guide_mix_pdf(o40, w):
+    o40 = np.ascontiguousarray(o40, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    return lib().orc_guide_mix_pdf(o40.ctypes.data_as(_d), w.ctypes.data_as(_d))
+
+
+def guide_random_params(seed, scale=1.0):
+    """Random network parameters with the padding (inputs 102.., outputs 40..)
+    zero, as the renderer keeps them."""
+    g = np.random.default_rng(seed)
+    w = np.zeros(GUIDE_PARAMS, np.float32)
+    o = 0
+    W1 = g.uniform(-0.2, 0.2, (128, 64)) * scale
+    W1[102:] = 0
+    b1 = g.uniform(-0.1, 0.1, 64) * scale
+    W2 = g.uniform(-0.2, 0.2, (64, 64)) * scale
+    b2 = g.uniform(-0.1, 0.1, 64) * scale
+    W3 = g.uniform(-0.3, 0.3, (64, 48)) * scale
+    W3[:, 40:] = 0
+    b3 = g.uniform(-0.5, 0.5, 48)
+    b3[40:] = 0
+    for a in (W1, b1, W2, b2, W3, b3):
+        a = np.asarray(a, np.float32).ravel()
+        w[o:o + a.size] = a
+        o += a.size
+    return w
