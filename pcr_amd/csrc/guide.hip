@@ -58,36 +58,79 @@ __device__ __forceinline__ void encode_row(const GuideNet &g, V3 p, V3 n, uint16
     for (int k = kGInUsed; k < kGIn; k++) row[k] = 0;
 }
 
-// LDS working set of one wave (64 rows).
+// LDS working set of one wave (64 rows) and the block's copy of the weights.
+// Rows are padded by 4 BF16 so the 16 rows a fragment load touches fall in
+// different banks; every fragment is one 8-byte load (4 consecutive k).
+constexpr int kLdX = kGIn + 4, kLdH = kGHid + 4;
 struct WaveMem {
-    uint16_t x[64 * kGIn];       // encoded inputs (BF16)
-    uint16_t h1[64 * kGHid];     // relu(layer 1) (BF16)
-    uint16_t h2[64 * kGHid];     // relu(layer 2) (BF16)
-    float o[64 * kGOutPad];      // layer 3 outputs (FP32)
+    alignas(16) uint16_t x[64 * kLdX];   // encoded inputs (BF16)
+    alignas(16) uint16_t h1[64 * kLdH];  // relu(layer 1) (BF16)
+    alignas(16) uint16_t h2[64 * kLdH];  // relu(layer 2) (BF16)
+    float o[64 * kGOutPad];              // layer 3 outputs (FP32)
 };
 struct WaveGrad {
     uint16_t dout[64 * kGOutPad];  // dLoss/dOutput (BF16)
     uint16_t da2[64 * kGHid];      // dLoss/d(pre-activation 2)
     uint16_t da1[64 * kGHid];      // dLoss/d(pre-activation 1)
 };
+struct NetLds {  // transposed weights [out][in]
+    alignas(16) uint16_t w1t[kGHid * kLdX];
+    alignas(16) uint16_t w2t[kGHid * kLdH];
+    alignas(16) uint16_t w3t[kGOutPad * kLdH];
+};
 
-// Forward pass of the wave's 64 rows (x already in LDS).  W*t are the BF16
-// copies stored transposed ([out][in]) so a B fragment is 4 adjacent values.
-__device__ __forceinline__ void net_forward(const GuideNet &g, WaveMem &M) {
-    const uint16_t *w1t = g.wb + (kGW1 + kGW2 + kGW3);
-    const uint16_t *w2t = w1t + kGW1, *w3t = w2t + kGW2;
+// Block-cooperative copy of the transposed BF16 weights into LDS.
+__device__ __forceinline__ void load_weights(const GuideNet &g, NetLds &W) {
+    const uint16_t *t1 = g.wb + (kGW1 + kGW2 + kGW3);
+    const uint16_t *t2 = t1 + kGW1, *t3 = t2 + kGW2;
+    for (int i = threadIdx.x; i < kGHid * kGIn / 4; i += blockDim.x) {
+        const int r = (4 * i) / kGIn, c = (4 * i) % kGIn;
+        *(uint2 *)(W.w1t + r * kLdX + c) = *(const uint2 *)(t1 + 4 * i);
+    }
+    for (int i = threadIdx.x; i < kGHid * kGHid / 4; i += blockDim.x) {
+        const int r = (4 * i) / kGHid, c = (4 * i) % kGHid;
+        *(uint2 *)(W.w2t + r * kLdH + c) = *(const uint2 *)(t2 + 4 * i);
+    }
+    for (int i = threadIdx.x; i < kGOutPad * kGHid / 4; i += blockDim.x) {
+        const int r = (4 * i) / kGHid, c = (4 * i) % kGHid;
+        *(uint2 *)(W.w3t + r * kLdH + c) = *(const uint2 *)(t3 + 4 * i);
+    }
+    __syncthreads();
+}
+
+// C[M x N] = A[M x K] * B[K x N] with A row-major in LDS (leading dimension
+// lda) and B given transposed, Bt[n][k] in LDS (ldb): both fragments are
+// single 8-byte LDS loads.
+template <int M, int N, int K, class FC>
+__device__ __forceinline__ void wave_gemm_lds(const uint16_t *A, int lda, const uint16_t *Bt, int ldb, FC out) {
+    const int l = (int)__lane_id();
+    const int r = l & 15, q = l >> 4;
+    for (int mt = 0; mt < M / 16; mt++)
+        for (int nt = 0; nt < N / 16; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            _Pragma("unroll") for (int kt = 0; kt < K / 16; kt++) {
+                const bf16x4 a = *(const bf16x4 *)(A + (mt * 16 + r) * lda + kt * 16 + 4 * q);
+                const bf16x4 b = *(const bf16x4 *)(Bt + (nt * 16 + r) * ldb + kt * 16 + 4 * q);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            }
+            _Pragma("unroll") for (int i = 0; i < 4; i++) out(mt * 16 + 4 * q + i, nt * 16 + r, acc[i]);
+        }
+}
+
+// Forward pass of the wave's 64 rows (x already in LDS).
+__device__ __forceinline__ void net_forward(const GuideNet &g, const NetLds &W, WaveMem &M) {
     const float *b1 = g.w + kGOffB1, *b2 = g.w + kGOffB2, *b3 = g.w + kGOffB3;
-    wave_gemm<64, kGHid, kGIn>([&](int m, int k) { return M.x[m * kGIn + k]; },
-                               [&](int k, int n) { return w1t[n * kGIn + k]; },
-                               [&](int m, int n, float v) { M.h1[m * kGHid + n] = f32_to_bf16(smax(0.f, fadd(v, b1[n]))); });
+    wave_gemm_lds<64, kGHid, kGIn>(M.x, kLdX, W.w1t, kLdX, [&](int m, int n, float v) {
+        M.h1[m * kLdH + n] = f32_to_bf16(smax(0.f, fadd(v, b1[n])));
+    });
     __syncthreads();
-    wave_gemm<64, kGHid, kGHid>([&](int m, int k) { return M.h1[m * kGHid + k]; },
-                                [&](int k, int n) { return w2t[n * kGHid + k]; },
-                                [&](int m, int n, float v) { M.h2[m * kGHid + n] = f32_to_bf16(smax(0.f, fadd(v, b2[n]))); });
+    wave_gemm_lds<64, kGHid, kGHid>(M.h1, kLdH, W.w2t, kLdH, [&](int m, int n, float v) {
+        M.h2[m * kLdH + n] = f32_to_bf16(smax(0.f, fadd(v, b2[n])));
+    });
     __syncthreads();
-    wave_gemm<64, kGOutPad, kGHid>([&](int m, int k) { return M.h2[m * kGHid + k]; },
-                                   [&](int k, int n) { return w3t[n * kGHid + k]; },
-                                   [&](int m, int n, float v) { M.o[m * kGOutPad + n] = fadd(v, b3[n]); });
+    wave_gemm_lds<64, kGOutPad, kGHid>(M.h2, kLdH, W.w3t, kLdH, [&](int m, int n, float v) {
+        M.o[m * kGOutPad + n] = fadd(v, b3[n]);
+    });
     __syncthreads();
 }
 
@@ -152,7 +195,10 @@ __device__ __forceinline__ uint32_t record_hash(uint32_t path, uint32_t bounce) 
 // Draw order from the path's stream: select, lobe, u1, u2.
 __global__ __launch_bounds__(64) void k_guide(GuideArgs a) {
     __shared__ WaveMem M;
+    __shared__ NetLds W;
     const uint32_t n = min(*a.count, a.cap);
+    if (blockIdx.x * 64u >= n) return;
+    load_weights(a.net, W);
     const int lane = (int)__lane_id();
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t i = base + (uint32_t)lane;
@@ -163,9 +209,9 @@ __global__ __launch_bounds__(64) void k_guide(GuideArgs a) {
             pend = (ps.meta & ST_GUIDE) != 0;
         }
         if (!__ballot(pend)) continue;  // uniform: the whole wave skips
-        encode_row(a.net, pend ? ps.o : v3(0.f, 0.f, 0.f), pend ? ps.d : v3(0.f, 0.f, 1.f), M.x + lane * kGIn);
+        encode_row(a.net, pend ? ps.o : v3(0.f, 0.f, 0.f), pend ? ps.d : v3(0.f, 0.f, 1.f), M.x + lane * kLdX);
         __syncthreads();
-        net_forward(a.net, M);
+        net_forward(a.net, W, M);
         if (pend) {
             const float *o = M.o + lane * kGOutPad;
             bool net_ok = true;
@@ -235,7 +281,13 @@ __global__ __launch_bounds__(64) void k_guide(GuideArgs a) {
 // split into S equal steps; wsum[s] collects each step's weight total.
 __global__ void k_guide_prep(GuideNet g, GuideRecord *rec, uint32_t rec_cap, const float4 *radiance, int S) {
     const uint32_t n = min(*g.n_rec, rec_cap);
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // whole waves iterate together so the per-step sums reduce across the wave
+    for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < n; r0 += stride) {
+        const uint32_t r = r0 + threadIdx.x;
+        float wt = 0.f;
+        int s = 0;
+        if (r < n) {
         GuideRecord &R = rec[r];
         const float4 f4 = radiance[R.path];
         const float fin[4] = {f4.x, f4.y, f4.z, f4.w};
@@ -249,10 +301,19 @@ __global__ void k_guide_prep(GuideNet g, GuideRecord *rec, uint32_t rec_cap, con
             }
         L = cnt ? fmaxf(L / (float)cnt, 0.f) : 0.f;
         const float cosw = R.n[0] * R.w[0] + R.n[1] * R.w[1] + R.n[2] * R.w[2];
-        const float wt = L * fmaxf(cosw, 0.f) / R.pdf;
-        R.weight = isfinite(wt) ? wt : 0.f;
-        const int s = (int)((uint64_t)r * (uint64_t)S / n);
-        atomicAdd(g.wsum + 2 * s, R.weight);
+        wt = L * fmaxf(cosw, 0.f) / R.pdf;
+        wt = isfinite(wt) ? wt : 0.f;
+        R.weight = wt;
+        s = (int)((uint64_t)r * (uint64_t)S / n);
+        }
+        const int s0 = __shfl(s, 0, 64);
+        if (__ballot(s != s0 && r < n) == 0) {  // one step in the wave: reduce, one atomic
+            float t = wt;
+            for (int dd = 32; dd >= 1; dd >>= 1) t += __shfl_xor(t, dd, 64);
+            if (__lane_id() == 0 && t != 0.f) atomicAdd(g.wsum + 2 * s0, t);
+        } else if (r < n && wt != 0.f) {
+            atomicAdd(g.wsum + 2 * s, wt);
+        }
     }
 }
 
@@ -296,10 +357,12 @@ __global__ __launch_bounds__(64) void k_guide_grad(GuideNet g, const GuideRecord
                                                    int S) {
     __shared__ WaveMem M;
     __shared__ WaveGrad D;
+    __shared__ NetLds Wl;
     const uint32_t n = min(*g.n_rec, rec_cap);
     const uint32_t r0 = (uint32_t)((uint64_t)s * n / S), r1 = (uint32_t)((uint64_t)(s + 1) * n / S);
     const float W = g.wsum[2 * s];
-    if (!(W > 0.f)) return;
+    if (!(W > 0.f) || r0 + blockIdx.x * 64u >= r1) return;
+    load_weights(g, Wl);
     const int lane = (int)__lane_id();
     const uint16_t *w2 = g.wb + kGW1, *w3 = w2 + kGW2;  // [in][out] layouts
     float *gw1 = g.grad + kGOffW1, *gb1 = g.grad + kGOffB1, *gw2 = g.grad + kGOffW2, *gb2 = g.grad + kGOffB2;
@@ -308,9 +371,9 @@ __global__ __launch_bounds__(64) void k_guide_grad(GuideNet g, const GuideRecord
         const uint32_t r = base + (uint32_t)lane;
         const bool ok = r < r1 && rec[r].weight > 0.f;
         GuideRecord R = rec[ok ? r : r0];
-        encode_row(g, v3(R.o[0], R.o[1], R.o[2]), v3(R.n[0], R.n[1], R.n[2]), M.x + lane * kGIn);
+        encode_row(g, v3(R.o[0], R.o[1], R.o[2]), v3(R.n[0], R.n[1], R.n[2]), M.x + lane * kLdX);
         __syncthreads();
-        net_forward(g, M);
+        net_forward(g, Wl, M);
         float d[kGOutPad];
         if (ok) {
             const float loss = output_grad(M.o + lane * kGOutPad, v3(R.w[0], R.w[1], R.w[2]), R.weight / W, d);
@@ -324,10 +387,10 @@ __global__ __launch_bounds__(64) void k_guide_grad(GuideNet g, const GuideRecord
         wave_gemm<64, kGHid, kGOutPad>([&](int m, int k) { return D.dout[m * kGOutPad + k]; },
                                        [&](int k, int nn) { return w3[nn * kGOutPad + k]; },
                                        [&](int m, int nn, float v) {
-                                           D.da2[m * kGHid + nn] = M.h2[m * kGHid + nn] ? f32_to_bf16(v) : (uint16_t)0;
+                                           D.da2[m * kGHid + nn] = M.h2[m * kLdH + nn] ? f32_to_bf16(v) : (uint16_t)0;
                                        });
         // dW3 = h2^T dOut
-        wave_gemm<kGHid, kGOutPad, 64>([&](int m, int k) { return M.h2[k * kGHid + m]; },
+        wave_gemm<kGHid, kGOutPad, 64>([&](int m, int k) { return M.h2[k * kLdH + m]; },
                                        [&](int k, int nn) { return D.dout[k * kGOutPad + nn]; },
                                        [&](int m, int nn, float v) { if (v != 0.f) atomicAdd(gw3 + m * kGOutPad + nn, v); });
         __syncthreads();
@@ -335,15 +398,15 @@ __global__ __launch_bounds__(64) void k_guide_grad(GuideNet g, const GuideRecord
         wave_gemm<64, kGHid, kGHid>([&](int m, int k) { return D.da2[m * kGHid + k]; },
                                     [&](int k, int nn) { return w2[nn * kGHid + k]; },
                                     [&](int m, int nn, float v) {
-                                        D.da1[m * kGHid + nn] = M.h1[m * kGHid + nn] ? f32_to_bf16(v) : (uint16_t)0;
+                                        D.da1[m * kGHid + nn] = M.h1[m * kLdH + nn] ? f32_to_bf16(v) : (uint16_t)0;
                                     });
         // dW2 = h1^T dA2
-        wave_gemm<kGHid, kGHid, 64>([&](int m, int k) { return M.h1[k * kGHid + m]; },
+        wave_gemm<kGHid, kGHid, 64>([&](int m, int k) { return M.h1[k * kLdH + m]; },
                                     [&](int k, int nn) { return D.da2[k * kGHid + nn]; },
                                     [&](int m, int nn, float v) { if (v != 0.f) atomicAdd(gw2 + m * kGHid + nn, v); });
         __syncthreads();
         // dW1 = x^T dA1
-        wave_gemm<kGIn, kGHid, 64>([&](int m, int k) { return M.x[k * kGIn + m]; },
+        wave_gemm<kGIn, kGHid, 64>([&](int m, int k) { return M.x[k * kLdX + m]; },
                                    [&](int k, int nn) { return D.da1[k * kGHid + nn]; },
                                    [&](int m, int nn, float v) { if (v != 0.f) atomicAdd(gw1 + m * kGHid + nn, v); });
         // biases: column sums of the (BF16) output and pre-activation gradients
@@ -409,14 +472,17 @@ __global__ void k_guide_sync(GuideNet g) {
 // Forward pass on given vertices (verification entry): out[n][kGOut].
 __global__ __launch_bounds__(64) void k_guide_forward(GuideNet g, const float *pn, uint32_t n, float *out) {
     __shared__ WaveMem M;
+    __shared__ NetLds W;
+    if (blockIdx.x * 64u >= n) return;
+    load_weights(g, W);
     const int lane = (int)__lane_id();
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t i = base + (uint32_t)lane;
         const uint32_t j = i < n ? i : 0u;
         encode_row(g, v3(pn[6 * j], pn[6 * j + 1], pn[6 * j + 2]), v3(pn[6 * j + 3], pn[6 * j + 4], pn[6 * j + 5]),
-                   M.x + lane * kGIn);
+                   M.x + lane * kLdX);
         __syncthreads();
-        net_forward(g, M);
+        net_forward(g, W, M);
         if (i < n)
             for (int k = 0; k < kGOut; k++) out[(size_t)i * kGOut + k] = M.o[lane * kGOutPad + k];
         __syncthreads();

@@ -1,0 +1,95 @@
+"""Learned-proposal benchmark: equal-time relMSE of guided vs plain sampling on
+the closed box with the small 30x30 light and a dispersive glass sphere
+(closed-c4), 1024x1024, depth 8, 4 shadow rays, spectral.
+
+  reference  plain sampling at 65536 samples per pixel over a fixed 256x256
+             crop (the same per-pixel sample streams as the full frame);
+  plain      the full frame at --spp samples, timed;
+  guided     the full frame at --spp samples with the online-trained
+             proposal (training included in the time), timed.
+
+relMSE is taken over the crop.  With relMSE proportional to 1 / samples, the
+time guided sampling needs to reach plain sampling's relMSE, relative to plain
+sampling's time, is (relMSE_guided / relMSE_plain) * (time_guided / time_plain):
+below 1 the proposal pays for itself.
+
+    python -m pcr_amd.guiding [--spp 64] [--ref-spp 65536]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import Context, HostScene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CROP = (384, 384, 256, 256)
+
+
+def relmse(img, ref, eps=1e-2):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def render(ds, crop=None, **kw):
+    t0 = time.perf_counter()
+    j = ds.submit(crop=crop, **kw)
+    j.wait()
+    dt = time.perf_counter() - t0
+    s = j.sums() / kw["spp"]
+    losses = j.guide_losses() if kw.get("guided") else None
+    j.release()
+    return s, dt, losses
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--ref-spp", type=int, default=65536)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--repeats", type=int, default=2)
+    a = ap.parse_args(argv)
+    ctx = Context(0)
+    ds = ctx.upload(HostScene(os.path.join(HERE, "scenes", "closed-c4.json")))
+    base = dict(width=a.size, height=a.size, depth=8, shadow=4, hero=4, spectral=True)
+    x0, y0, cw, ch = CROP if a.size == 1024 else (a.size // 4, a.size // 4, a.size // 4, a.size // 4)
+    t0 = time.perf_counter()
+    ref, _, _ = render(ds, crop=(x0, y0, cw, ch), spp=a.ref_spp, seed=987654321, **base)
+    t_ref = time.perf_counter() - t0
+    print("reference: %d spp over the %dx%d crop in %.1f s" % (a.ref_spp, cw, ch, t_ref), flush=True)
+    render(ds, spp=4, seed=1, **base)  # warm-up (queues, code objects)
+    res = {}
+    for name, extra in (("plain", {}), ("guided", {"guided": True})):
+        errs, times, losses = [], [], None
+        for r in range(a.repeats):
+            img, dt, ls = render(ds, spp=a.spp, seed=1000 + 17 * r, **base, **extra)
+            errs.append(relmse(img[y0:y0 + ch, x0:x0 + cw], ref))
+            times.append(dt)
+            losses = ls if ls is not None else losses
+        res[name] = dict(relmse=float(np.mean(errs)), seconds=float(np.mean(times)))
+        if losses is not None:
+            res[name]["loss_first_last"] = [round(float(losses[0]), 4), round(float(losses[-1]), 4)]
+        print(name, res[name], flush=True)
+    ds.release()
+    ctx.close()
+    v = res["guided"]["relmse"] / res["plain"]["relmse"]
+    t = res["guided"]["seconds"] / res["plain"]["seconds"]
+    out = {
+        "metric": "equal-time relMSE guided/plain (closed box, 30x30 light, glass sphere, %d^2, d8)" % a.size,
+        "value": round(v * t, 4),
+        "unit": "ratio (lower is better; guided reaches plain's relMSE in this fraction of the time)",
+        "equal_spp_relmse_ratio": round(v, 4),
+        "time_ratio": round(t, 4),
+        "spp": a.spp,
+        "reference": {"spp": a.ref_spp, "crop": [x0, y0, cw, ch], "seconds": round(t_ref, 1)},
+        "plain": res["plain"],
+        "guided": res["guided"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
