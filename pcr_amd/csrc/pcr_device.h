@@ -81,6 +81,60 @@ struct DScene {
     float fov;
 };
 
+// ------------------------------------------------------ global accesses ---
+// Scene and job buffers are reached through pointers read from the job table,
+// which the compiler can only treat as generic (flat) pointers; flat accesses
+// also count against the LDS counter, so each one stalls the LDS traversal
+// stack as well.  These helpers load and store through the global address
+// space (vector types, so no constructor needs an address-space reference).
+typedef float pcr_f4_t __attribute__((ext_vector_type(4)));
+typedef int pcr_i2_t __attribute__((ext_vector_type(2)));
+#define PCR_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ float4 ldg(const float4 *p) {
+    const pcr_f4_t v = *(const PCR_GLOBAL pcr_f4_t *)(const void *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int2 ldg(const int2 *p) {
+    const pcr_i2_t v = *(const PCR_GLOBAL pcr_i2_t *)(const void *)p;
+    return make_int2(v.x, v.y);
+}
+__device__ __forceinline__ float ldg(const float *p) { return *(const PCR_GLOBAL float *)(const void *)p; }
+__device__ __forceinline__ int32_t ldg(const int32_t *p) { return *(const PCR_GLOBAL int32_t *)(const void *)p; }
+__device__ __forceinline__ double ldg(const double *p) { return *(const PCR_GLOBAL double *)(const void *)p; }
+__device__ __forceinline__ void stg(float4 *p, float4 v) {
+    *(PCR_GLOBAL pcr_f4_t *)(void *)p = pcr_f4_t{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void stg(float *p, float v) { *(PCR_GLOBAL float *)(void *)p = v; }
+__device__ __forceinline__ void stg(double *p, double v) { *(PCR_GLOBAL double *)(void *)p = v; }
+__device__ __forceinline__ DMat ldg(const DMat *p) {
+    const float4 *q = (const float4 *)p;
+    DMat m;
+    m.albedo = ldg(q);
+    m.emissive = ldg(q + 1);
+    m.fitA = ldg(q + 2);
+    m.fitE = ldg(q + 3);
+    const float4 e = ldg(q + 4);
+    m.flags = __float_as_uint(e.x);
+    m.tab = __float_as_int(e.y);
+    m.pad0 = m.pad1 = 0;
+    m.pad2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    return m;
+}
+__device__ __forceinline__ DLight ldg(const DLight *p) {
+    const float4 *q = (const float4 *)p;
+    const float4 h = ldg(q);
+    DLight L;
+    L.kind = __float_as_int(h.x);
+    L.first = __float_as_int(h.y);
+    L.count = __float_as_int(h.z);
+    L.area = h.w;
+    L.o = ldg(q + 1);
+    L.u = ldg(q + 2);
+    L.v = ldg(q + 3);
+    L.n = ldg(q + 4);
+    return L;
+}
+
 // ------------------------------------------------------------- vec math ----
 // Plain operators: with -ffp-contract=off nothing is fused, and with
 // -fhip-fp32-correctly-rounded-divide-sqrt division and sqrtf round
@@ -191,9 +245,9 @@ __device__ __forceinline__ float spec_eval(const float *s, float l) {
     if (l < 400.f || l > 700.f) return 0.f;
     float t = div_rn(fsub(l, 400.f), 5.f);
     int i = (int)t;
-    if (i >= 60) return s[60];
+    if (i >= 60) return ldg(s + 60);
     float f = fsub(t, (float)i);
-    return fadd(fmul(s[i], fsub(1.f, f)), fmul(s[i + 1], f));
+    return fadd(fmul(ldg(s + i), fsub(1.f, f)), fmul(ldg(s + i + 1), f));
 }
 __device__ __forceinline__ float sigmoid(float x) {
     return fadd(0.5f, div_rn(x, fmul(2.f, sqrt_rn(fadd(1.f, fmul(x, x))))));
@@ -219,7 +273,7 @@ __device__ __forceinline__ float cauchy_ior(float ior, float B, float l) {
 // Moller-Trumbore; on success updates closest and the (triangle, u, v) record.
 __device__ __forceinline__ bool tri_test(const float4 *__restrict__ tri, int ti, V3 o, V3 d, float &closest,
                                          int &best, float &bu, float &bv) {
-    float4 a = tri[3 * ti], b = tri[3 * ti + 1], c = tri[3 * ti + 2];
+    float4 a = ldg(tri + 3 * ti), b = ldg(tri + 3 * ti + 1), c = ldg(tri + 3 * ti + 2);
     V3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
@@ -302,7 +356,7 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
     h.u = h.v = 0.f;
     float closest = tmax;
     for (int i = 0; i < S.n_sph; i++) {
-        float4 s = S.sph[i];
+        float4 s = ldg(S.sph + i);
         V3 cp = sub(xyz(s), o);
         float rl = dot(d, cp);
         float tSq = fsub(dot(cp, cp), fmul(rl, rl));
@@ -319,7 +373,8 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
         h.prim = i;
     }
     for (int i = 0; i < S.n_pl; i++) {
-        float4 po = S.pl[4 * i], pu = S.pl[4 * i + 1], pv = S.pl[4 * i + 2], pn = S.pl[4 * i + 3];
+        float4 po = ldg(S.pl + 4 * i), pu = ldg(S.pl + 4 * i + 1), pv = ldg(S.pl + 4 * i + 2),
+               pn = ldg(S.pl + 4 * i + 3);
         V3 N = xyz(pn);
         float denom = dot(d, N);
         if (fabsf(denom) <= 1e-6f) continue;
@@ -346,11 +401,11 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
             int cur = S.root_ref;
             for (;;) {
                 if (cur < 0) {
-                    int2 lf = S.leaves[~cur];
+                    int2 lf = ldg(S.leaves + ~cur);
                     for (int k = 0; k < lf.y; k++) tri_test(S.tri, lf.x + k, o, d, closest, best, bu, bv);
                 } else {
                     const float4 *nd = S.bvh + 4 * cur;
-                    float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+                    float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), e = ldg(nd + 3);
                     float tL, tR;
                     bool hL = box_test(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, closest, tL);
                     bool hR = box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, closest, tR);
@@ -401,24 +456,24 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
 __device__ __forceinline__ void hit_geometry(const DScene &S, const HitRec &h, V3 o, V3 d, V3 &P, V3 &N, int &mat) {
     P = add(o, mul(d, h.t));
     if (h.kind == 0) {
-        N = normalize(sub(P, xyz(S.sph[h.prim])));
-        mat = S.sph_mat[h.prim];
+        N = normalize(sub(P, xyz(ldg(S.sph + h.prim))));
+        mat = ldg(S.sph_mat + h.prim);
     } else if (h.kind == 1) {
-        N = xyz(S.pl[4 * h.prim + 3]);
-        mat = __float_as_int(S.pl[4 * h.prim + 2].w);
+        N = xyz(ldg(S.pl + 4 * h.prim + 3));
+        mat = __float_as_int(ldg(S.pl + 4 * h.prim + 2).w);
     } else {
-        float4 a = S.tri[3 * h.prim], b = S.tri[3 * h.prim + 1];
+        float4 a = ldg(S.tri + 3 * h.prim), b = ldg(S.tri + 3 * h.prim + 1);
         mat = __float_as_int(a.w);
         const float4 *tn = S.tri_n + 4 * h.prim;
         if (__float_as_int(b.w)) {
             float w = fsub(fsub(1.f, h.u), h.v);
-            float4 n0 = tn[0], n1 = tn[1], n2 = tn[2];
+            float4 n0 = ldg(tn), n1 = ldg(tn + 1), n2 = ldg(tn + 2);
             V3 in = v3(fadd(fadd(fmul(n0.x, w), fmul(n1.x, h.u)), fmul(n2.x, h.v)),
                        fadd(fadd(fmul(n0.y, w), fmul(n1.y, h.u)), fmul(n2.y, h.v)),
                        fadd(fadd(fmul(n0.z, w), fmul(n1.z, h.u)), fmul(n2.z, h.v)));
             N = normalize(in);
         } else {
-            N = xyz(tn[3]);
+            N = xyz(ldg(tn + 3));
         }
     }
 }

@@ -131,9 +131,9 @@ __device__ __forceinline__ void write_radiance(const DJob &J, const PathState &p
     int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
     float *r = (float *)(J.radiance + ps.path);
     if (ch0 == 0 && nch >= 3) {
-        J.radiance[ps.path] = make_float4(ps.rad[0], ps.rad[1], ps.rad[2], ps.rad[3]);
+        stg(J.radiance + ps.path, make_float4(ps.rad[0], ps.rad[1], ps.rad[2], ps.rad[3]));
     } else {
-        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) r[ch0 + c] = ps.rad[c];
+        _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) stg(r + ch0 + c, ps.rad[c]);
     }
 }
 
@@ -191,7 +191,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
     hit_geometry(S, h, ps.o, ps.d, hit, N, mi);
     bool entering = dot(ps.d, N) < 0.f;
     if (!entering) N = mul(N, -1.f);
-    const DMat &M = S.mat[mi];
+    const DMat M = ldg(S.mat + mi);
     if (M.flags & MF_EMISSIVE) {
         cnt.emit++;
         _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) {
@@ -286,11 +286,11 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
                 float target = fmul(rng_next(ps.rng), S.total_area);
                 float cum = 0.f;
                 for (int k = 0; k < S.n_lights; k++) {
-                    cum = fadd(cum, S.lights[k].area);
+                    cum = fadd(cum, ldg(&S.lights[k].area));
                     if (target <= cum) { li = k; break; }
                 }
             }
-            const DLight &L = S.lights[li];
+            const DLight L = ldg(S.lights + li);
             V3 sP, sN;
             int lmat;
             if (L.kind == 0) {
@@ -303,7 +303,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
                 int lo = 0, hi = L.count;
                 while (lo < hi) {  // std::lower_bound on the cumulative areas
                     int mid = (lo + hi) >> 1;
-                    if (S.ltri[4 * (L.first + mid) + 1].w < rt) lo = mid + 1;
+                    if (ldg(S.ltri + 4 * (L.first + mid) + 1).w < rt) lo = mid + 1;
                     else hi = mid;
                 }
                 int ti = L.first + (lo < L.count - 1 ? lo : L.count - 1);
@@ -313,7 +313,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
                     r2 = fsub(1.f, r2);
                 }
                 const float4 *lt = S.ltri + 4 * ti;
-                float4 a = lt[0], e1 = lt[1], e2 = lt[2], fn = lt[3];
+                float4 a = ldg(lt), e1 = ldg(lt + 1), e2 = ldg(lt + 2), fn = ldg(lt + 3);
                 sP = add(add(xyz(a), mul(xyz(e1), r1)), mul(xyz(e2), r2));
                 sN = xyz(fn);
                 lmat = __float_as_int(a.w);
@@ -337,7 +337,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
                 int sm;
                 hit_geometry(S, sh, so, wi, sp, sn, sm);
                 V3 dd = sub(sp, so);
-                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(S.mat[sm].flags & MF_EMISSIVE);
+                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(ldg(S.mat + sm).flags & MF_EMISSIVE);
             }
             if (!occl) {
                 float cosL = smax(0.f, dot(sN, mul(Li, -1.f)));
@@ -349,7 +349,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
                     w = div_rn(fmul(pl, pl), fadd(fmul(pl, pl), fmul(pb, pb)));
                 }
                 if (lmat != le_mat) {
-                    const DMat &LM = S.mat[lmat];
+                    const DMat LM = ldg(S.mat + lmat);
                     _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch)
                         le_c[c] = spectral ? emissive_at(LM, S.tabs, lam[c])
                                            : (c == 0 ? LM.emissive.x : (c == 1 ? LM.emissive.y : LM.emissive.z));
@@ -581,12 +581,12 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
         if (slot < 0) continue;
         const DJob &J = a.jobs[slot];
         double *dst = J.accum + 3 * (size_t)pix;
-        double acc0 = dst[0], acc1 = dst[1], acc2 = dst[2];
+        double acc0 = ldg(dst), acc1 = ldg(dst + 1), acc2 = ldg(dst + 2);
         const bool spectral = (J.flags & JF_SPECTRAL) != 0;
         const int N = J.hero;
         for (int sl = 0; sl < J.spp_pass; sl++) {
             uint32_t path = pix * (uint32_t)J.spp_pass + (uint32_t)sl;
-            float4 r = J.radiance[path];
+            float4 r = ldg(J.radiance + path);
             float o0, o1, o2;
             if (spectral) {
                 int px, py, s;
@@ -615,9 +615,9 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
             acc1 += (double)o1;
             acc2 += (double)o2;
         }
-        dst[0] = acc0;
-        dst[1] = acc1;
-        dst[2] = acc2;
+        stg(dst, acc0);
+        stg(dst + 1, acc1);
+        stg(dst + 2, acc2);
     }
 }
 
