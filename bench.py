@@ -17,7 +17,21 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-WORKLOAD = dict(scene="cornell-c1", width=720, height=720, depth=8, spp=64, shadow=4, hero=4)
+WORKLOADS = {
+    # the headline: 720x720 Cornell box with a mirror and a dispersive glass sphere
+    "c1": dict(scene="cornell-c1", width=720, height=720, depth=8, spp=64, shadow=4, hero=4,
+               metric="Mpaths/s 720^2 Cornell d8 (mirror + Cauchy glass spheres, 64 spp, 4 shadow rays, spectral)",
+               desc="cornell-c1 720x720 depth 8, 64 spp, 4 shadow rays, 4 hero wavelengths",
+               data="synthetic scene (cornell-c1: 555-unit box, two r=90 spheres), per-step seeds"),
+    # traversal stress: the box with an 81,932-triangle displaced icosphere
+    "c2": dict(scene="cornell-c2", width=1024, height=1024, depth=8, spp=256, shadow=4, hero=4,
+               metric="Mpaths/s 1024^2 Cornell d8 + 81,932-triangle displaced icosphere (256 spp, 4 shadow rays, "
+                      "spectral)",
+               desc="cornell-c2 1024x1024 depth 8, 256 spp, 4 shadow rays, 4 hero wavelengths",
+               data="synthetic scene (cornell-c2: seeded 3-octave value-noise icosphere, subdivision 6), "
+                    "per-step seeds"),
+}
+WORKLOAD = WORKLOADS["c1"]
 # Algorithmic bytes per unit of work (minimal SoA records, each written once
 # and read once across a kernel boundary): per path 64 B (state init + pixel
 # accumulate), per extension ray 176 B (ray 56 + hit 32 + state 72 + queue
@@ -45,6 +59,8 @@ def parse():
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--pass-paths", type=int, default=0)
     p.add_argument("--no-autotune", action="store_true", help="keep the library's default launch configuration")
+    p.add_argument("--workload", choices=sorted(WORKLOADS), default="c1",
+                   help="c1 (default, the headline metric) or c2 (the 81,932-triangle traversal workload)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run 3 device-to-device copies of MIB MiB each (the reference workload for "
                         "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -84,7 +100,7 @@ def copy_gbs(mib=1024, reps=5):
 
 def cpu_baseline(scene_path, target_s):
     """The CPU oracle's non-branching estimator on a bounded sample of the same
-    frame: the full 720x720 frame at the first k samples per pixel, k chosen
+    frame: the full frame at the first k samples per pixel, k chosen
     from a 1-sample calibration run so the sample takes about target_s; all
     host threads (at most 16, the box's CPU share)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -102,12 +118,15 @@ def cpu_baseline(scene_path, target_s):
     k = max(1, min(spp, int(round(target_s / max(t1, 1e-3)))))
     dt, paths = run(k)
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": "oracle non-branching estimator: the 720x720 frame at samples 0..%d of 64, depth %d, "
-                      "%d shadow rays, spectral; %d paths in %.1f s on %d threads" % (k - 1, d, S, paths, dt, threads)}
+            "sample": "oracle non-branching estimator: the %dx%d frame at samples 0..%d of %d, depth %d, "
+                      "%d shadow rays, spectral; %d paths in %.1f s on %d threads"
+                      % (W, H, k - 1, spp, d, S, paths, dt, threads)}
 
 
 def main():
+    global WORKLOAD
     a = parse()
+    WORKLOAD = WORKLOADS[a.workload]
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this pool has a single MI355X; "
               "multi-GPU runs are not supported by this benchmark.", file=sys.stderr)
@@ -159,7 +178,7 @@ def main():
     achieved = alg_bytes / (prof.bounce_ms * 1e-3) / 1e9 if prof.bounce_ms > 0 else None
     value = paths / dt / 1e6
     out = {
-        "metric": "Mpaths/s 720^2 Cornell d8 (mirror + Cauchy glass spheres, 64 spp, 4 shadow rays, spectral)",
+        "metric": WORKLOAD["metric"],
         "value": round(value, 3),
         "unit": "Mpaths/s",
         "n_gpus": 1,
@@ -170,8 +189,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic scene (cornell-c1: 555-unit box, two r=90 spheres), per-step seeds",
-        "config": {"workload": "cornell-c1 720x720 depth 8, 64 spp, 4 shadow rays, 4 hero wavelengths",
+        "data": WORKLOAD["data"],
+        "config": {"workload": WORKLOAD["desc"],
                    "paths_per_step": paths // a.steps, "ext_rays_per_step": ext // a.steps,
                    "shadow_rays_per_step": shadow // a.steps, "passes_per_step": stats[0].passes,
                    "launch": {"block": tuned.block, "blocks_per_cu": tuned.blocks_per_cu,
@@ -187,8 +206,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH if all(getattr(tuned, k) == v for k, v in TRAFFIC_CONFIG.items())
-            else None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if a.workload == "c1" and all(
+                getattr(tuned, k) == v for k, v in TRAFFIC_CONFIG.items()) else None,
             "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per k_bounce dispatch",
             "copy_gbs": round(copy_gbs(), 1),
             "launches": prof.bounce_launches,

@@ -400,9 +400,9 @@ void admit(pcrhip_ctx *c) {
     }
 }
 
-// The bounce kernel's traversal-stack variant (8, 16, 32 or 64 entries).
+// The bounce kernel's traversal-stack variant (8, 16, 24, 32 or 64 entries).
 int stack_class(int stack) {
-    return stack <= 8 ? 8 : stack <= 16 ? 16 : stack <= 32 ? 32 : 64;
+    return stack <= 8 ? 8 : stack <= 16 ? 16 : stack <= 24 ? 24 : stack <= 32 ? 32 : 64;
 }
 
 // Jobs that may share a pass: same stack variant; a guided job trains its own
@@ -904,7 +904,7 @@ int build_scene(pcrhip_scene *s, const pcrhip_scene_desc *d) {
     }
     s->depth = depth;
     // traversal stack: at most depth-1 pending far children below the current node
-    s->stack = depth <= 8 ? 8 : depth <= 16 ? 16 : depth <= 32 ? 32 : 64;
+    s->stack = depth <= 8 ? 8 : depth <= 16 ? 16 : depth <= 24 ? 24 : depth <= 32 ? 32 : 64;
     // lights
     std::vector<DLight> lights(d->n_lights);
     std::vector<float4> ltri(4 * (size_t)d->n_light_tris);

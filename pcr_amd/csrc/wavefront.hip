@@ -448,8 +448,14 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // rays in 128-ray chunks, so a wave almost always holds rays of one job: the
 // job's descriptor and scene are read with scalar loads and the shading code
 // runs once per wave.
+// Occupancy: 3 waves per SIMD (<= 168 VGPRs, a few dozen bytes of spills)
+// measured 20% faster than the unconstrained 2 waves at 185-195 VGPRs, and 4
+// waves (128 VGPRs, ~200 bytes of spills) slower.  Up to 24 stack entries,
+// 3 waves per SIMD also fit LDS (12 waves x 64 lanes x 24 x 8 bytes = 147 KB
+// of 160 KB); deeper stacks are limited by LDS and keep the compiler's choice.
 template <bool FIRST, int STACK, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bounce(BounceArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 24 ? 3 : 1, STACK <= 24 ? 3 : 10)))
+void k_bounce(BounceArgs a) {
     __shared__ int32_t s_idx[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
     __shared__ uint32_t s_base;
@@ -678,7 +684,7 @@ static hipError_t launch_bounce_s(const BounceArgs &a, bool first, int grid, int
 
 // LDS per block is STACK * BLOCK * 8 bytes: deep stacks use smaller blocks.
 extern "C" int pcr_bounce_block(int stack, int block) {
-    if (stack <= 16) return block;
+    if (stack <= 24) return block;
     if (stack <= 32) return block == 256 ? 128 : block;
     return 64;
 }
@@ -688,6 +694,7 @@ extern "C" hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int
     block = pcr_bounce_block(stack, block);
     if (stack <= 8) return launch_bounce_s<8>(*a, first != 0, grid, block, s);
     if (stack <= 16) return launch_bounce_s<16>(*a, first != 0, grid, block, s);
+    if (stack <= 24) return launch_bounce_s<24>(*a, first != 0, grid, block, s);
     if (stack <= 32) return launch_bounce_s<32>(*a, first != 0, grid, block, s);
     return launch_bounce_s<64>(*a, first != 0, grid, block, s);
 }
