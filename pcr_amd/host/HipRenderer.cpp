@@ -80,6 +80,7 @@ void HipRenderer::render(const Scene &scene, std::chrono::steady_clock::time_poi
     if (useRussian) fl |= PCRHIP_RUSSIAN;
     if (useStratified) fl |= PCRHIP_STRATIFIED;
     if (aa > 1) fl |= PCRHIP_JITTER;
+    if (useGuiding) fl |= PCRHIP_GUIDED;
     jd.flags = fl;
     // The reference GPU backends seed from the clock (OpenglRenderer.cpp:1995-1997).
     jd.seed = seed ? seed
@@ -160,6 +161,7 @@ void HipRenderer::render(const Scene &scene, std::chrono::steady_clock::time_poi
         {"OIDN", useOIDN ? "1" : "0"},
         {"Spectral", useSpectral ? "1" : "0"},
         {"WavefrontMode", "persistent-fused"},
+        {"Guided", useGuiding ? "1" : "0"},
         {"Seed", std::to_string(jd.seed)},
     };
     if (useSpectral) text.push_back({"HeroSamples", std::to_string(std::clamp(heroSamples, 1, 4))});

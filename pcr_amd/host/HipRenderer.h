@@ -55,6 +55,9 @@ public:
     // reference GPU backends do) and the device ordinal.
     uint64_t seed = 0;
     int device = 0;
+    // Extension: learned directional proposal at diffuse vertices (trained
+    // online during the render, MIS-combined 50/50 with cosine sampling).
+    bool useGuiding = false;
 
     void render(const Scene &scene, std::chrono::steady_clock::time_point start, const std::string &outputDir);
 

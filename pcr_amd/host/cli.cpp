@@ -84,6 +84,7 @@ std::vector<Found> discover(const std::vector<std::string> &extra) {
                  "  --adaptive --denoise --oidn   accepted; not provided by this backend\n"
                  "  --hero-samples N        hero wavelengths 1..4 (4)\n"
                  "  --seed N                NumGen seed (0: from the clock)\n"
+                 "  --guided                learned directional proposal (trained during the render)\n"
                  "  --device N              GPU ordinal (0)\n";
     std::exit(code);
 }
@@ -96,7 +97,7 @@ int main(int argc, char **argv) {
     std::string scene = "cornell", outputDir;
     std::vector<std::string> dirs;
     bool list = false, denoise = false, mis = false, russian = false, strat = false, aces = false, aa = false,
-         adaptive = false, oidn = false, spectral = false;
+         adaptive = false, oidn = false, spectral = false, guided = false;
     uint64_t seed = 0;
     // "--flag=value" is split into "--flag" "value" (CLI11 accepts both forms)
     std::vector<std::string> args;
@@ -159,6 +160,7 @@ int main(int argc, char **argv) {
             }
         } else if (a == "--seed") seed = std::strtoull(need(i).c_str(), nullptr, 10);
         else if (a == "--device") device = std::atoi(need(i).c_str());
+        else if (a == "--guided") guided = true;
         else if (a == "--tz" || a == "--timezone") setenv("TZ", need(i).c_str(), 1);
         else if (a == "--lut" || a == "--lut-file") {
             std::cerr << a << ": the RGB-to-spectrum lookup-table cache is not provided by this build\n";
@@ -205,6 +207,7 @@ int main(int argc, char **argv) {
     r.heroSamples = hero;
     r.seed = seed;
     r.device = device;
+    r.useGuiding = guided;
     r.render(sd, start, outputDir);
     return r.lastOutputPath.empty() ? 1 : 0;
 }
