@@ -190,8 +190,8 @@ typedef struct {
     int x0, y0, cw, ch;  /* crop window; cw = ch = 0 renders the full frame */
     int priority;        /* higher is admitted first */
     /* PCRHIP_GUIDED training per pass (0 selects the default): Adam steps
-     * (<= 16, default 16), learning rate (default 5e-3), training samples
-     * recorded (default 65536). */
+     * (<= 128, default 128), learning rate (default 5e-3), training samples
+     * recorded (default 262144). */
     int guide_steps;
     float guide_lr;
     int guide_records;

@@ -29,13 +29,100 @@ constexpr int kGW1 = kGIn * kGHid, kGW2 = kGHid * kGHid, kGW3 = kGHid * kGOutPad
 constexpr int kGOffW1 = 0, kGOffB1 = kGW1, kGOffW2 = kGOffB1 + kGHid, kGOffB2 = kGOffW2 + kGW2,
               kGOffW3 = kGOffB2 + kGHid, kGOffB3 = kGOffW3 + kGW3, kGParams = kGOffB3 + kGOutPad;
 constexpr float kGLogKappaMin = -3.f, kGLogKappaMax = 7.f;
-constexpr int kGMaxSteps = 16;  // training steps per pass
+constexpr int kGMaxSteps = 128;  // training steps per pass (at most)
+// Training defaults per pass: Adam steps (the records are split evenly over
+// them) and vertices recorded.
+constexpr int kGDefaultSteps = 128, kGDefaultRecords = 262144;
+
+// The inference kernels keep a wave's activations in registers: rays are the
+// N dimension of v_mfma_f32_16x16x32_bf16 (H^T = W^T X^T), so a layer's
+// accumulator tiles are, after bias, ReLU and BF16 rounding, the B operand of
+// the next layer.  The weights are stored once more as ready-made A operands:
+// fragment f holds, for each of the 64 lanes, the 8 BF16 values that lane
+// passes to one MFMA.  The K order inside a fragment follows what a lane holds
+// (permuting K changes nothing as long as A and B agree):
+//   layer 1: 4 K-blocks of 32 input slots.  Slot group (kb < 3, lane quarter q)
+//            is chain c = 4 kb + q: input dimension c % 6 at frequencies
+//            j = 4 (c / 6) .. 4 (c / 6) + 3, as (sin, cos) pairs; group
+//            (kb = 3, q = 0) holds the 6 raw inputs; the other groups are 0.
+//   layers 2, 3: K-block kb of 32 hidden units; lane quarter q holds units
+//            32 kb + 4 q + i and 32 kb + 16 + 4 q + i, i = 0..3 -- exactly the
+//            rows of the previous layer's two 16-row output tiles in that lane.
+// Fragments: layer 1 f = 4 mt + kb (16), layer 2 f = 16 + 2 mt + kb (8),
+// layer 3 f = 24 + 2 mt + kb (6), mt = 16-row block of the layer's outputs.
+constexpr int kGFrags = 30, kGFragElems = kGFrags * 64 * 8;
+// The backward pass multiplies by W3 and W2 untransposed: A-operand fragments
+// with rows = the layer's input units (16-row block mt) and K = its outputs,
+//   W3: f = 2 mt + kb, K slot (kb, q, i) = output 32 kb + 8 q + i (0 past 47);
+//   W2: f = 8 + 2 mt + kb, K slot (kb, q, i) = the layer-2 unit order above.
+constexpr int kGBackFrags = 16, kGBackElems = kGBackFrags * 64 * 8;
+// Training blocks per step: each writes one row of per-parameter gradient sums.
+constexpr int kGTrainBlocks = 64;
+
+// Offset in the fragment array of FP32 parameter p, or -1 (biases, padding
+// rows of W1 and anything no lane multiplies).
+__host__ __device__ inline int guide_frag_offset(int p) {
+    int f, q, i, m;
+    if (p >= kGOffW1 && p < kGOffB1) {
+        const int in = p / kGHid;
+        m = p % kGHid;
+        int kb;
+        if (in < 6) {
+            kb = 3; q = 0; i = in;
+        } else if (in < kGInUsed) {
+            const int mm = in - 6, j = mm / 12, d = (mm % 12) / 2, c = 6 * (j / 4) + d;
+            kb = c / 4; q = c % 4; i = 2 * (j % 4) + (mm & 1);
+        } else {
+            return -1;
+        }
+        f = 4 * (m / 16) + kb;
+    } else if ((p >= kGOffW2 && p < kGOffB2) || (p >= kGOffW3 && p < kGOffB3)) {
+        const bool l2 = p < kGOffB2;
+        const int x = l2 ? p - kGOffW2 : p - kGOffW3, w = l2 ? kGHid : kGOutPad;
+        const int u = x / w, uu = u % 32;
+        m = x % w;
+        q = (uu % 16) / 4;
+        i = 4 * (uu / 16) + uu % 4;
+        f = (l2 ? 16 : 24) + 2 * (m / 16) + u / 32;
+    } else {
+        return -1;
+    }
+    return (f * 64 + 16 * q + m % 16) * 8 + i;
+}
+__host__ __device__ inline int guide_back_offset(int p) {
+    int f, q, i, u;
+    if (p >= kGOffW3 && p < kGOffB3) {
+        const int x = p - kGOffW3, m = x % kGOutPad;
+        u = x / kGOutPad;
+        f = 2 * (u / 16) + m / 32;
+        q = (m % 32) / 8;
+        i = m % 8;
+    } else if (p >= kGOffW2 && p < kGOffB2) {
+        const int x = p - kGOffW2, m = x % kGHid, mm = m % 32;
+        u = x / kGHid;
+        f = 8 + 2 * (u / 16) + m / 32;
+        q = (mm % 16) / 4;
+        i = 4 * (mm / 16) + mm % 4;
+    } else {
+        return -1;
+    }
+    return (f * 64 + 16 * q + u % 16) * 8 + i;
+}
+// Input feature of layer-1 K slot 32 kb + 8 q + i, or -1 for a zero slot.
+__host__ __device__ inline int guide_slot_feature(int slot) {
+    const int kb = slot / 32, q = (slot % 32) / 8, i = slot % 8;
+    if (kb == 3) return (q == 0 && i < 6) ? i : -1;
+    const int c = 4 * kb + q, d = c % 6, j = 4 * (c / 6) + i / 2;
+    return 6 + 12 * j + 2 * d + (i & 1);
+}
 
 // Device state of one job's network.
 struct GuideNet {
     float *w;          // [kGParams] FP32 master weights
-    uint16_t *wb;      // [2 (kGW1 + kGW2 + kGW3)] BF16 copies of W1, W2, W3: [in][out], then [out][in]
-    float *grad;       // [kGParams] gradient accumulator
+    uint16_t *wf;      // [kGFragElems] BF16 weights as forward MFMA A-operand fragments
+    uint16_t *wk;      // [kGBackElems] BF16 W3, W2 as backward fragments
+    float *grad;       // [kGParams] gradient of the last step
+    float *part;       // [kGTrainBlocks][kGParams] per-block gradient sums of a step
     float *m, *v;      // Adam moments
     float *wsum;       // [2 * kGMaxSteps]: per training step, weight total and loss
     uint32_t *n_rec;   // records written this pass
@@ -60,6 +147,7 @@ static_assert(sizeof(GuideRecord) == 96, "record layout");
 struct GuideTrain {
     float lr;
     float beta1, beta2, eps;
+    float c1, c2;  // bias corrections of step t: 1 - beta1^t, 1 - beta2^t
 };
 
 // ------------------------------------------------------------ BF16 -------
@@ -82,6 +170,7 @@ struct GuideArgs {
     GuideRecord *rec;
     uint32_t rec_cap;
     uint32_t keep;          // record a vertex if its 24-bit hash is below keep
+    int mix;                // 0: the network is untrained, sample the cosine density alone
 };
 }  // namespace pcr
 
@@ -89,9 +178,9 @@ extern "C" {
 hipError_t pcr_launch_guide(const pcr::GuideArgs *a, int grid, hipStream_t s);
 hipError_t pcr_launch_guide_prep(const pcr::GuideNet *g, pcr::GuideRecord *rec, uint32_t rec_cap,
                                  const float4 *radiance, int S, hipStream_t s);
-hipError_t pcr_launch_guide_grad(const pcr::GuideNet *g, const pcr::GuideRecord *rec, uint32_t rec_cap, int step,
-                                 int S, int grid, hipStream_t s);
-hipError_t pcr_launch_guide_adam(const pcr::GuideNet *g, const pcr::GuideTrain *h, int t, hipStream_t s);
+hipError_t pcr_launch_guide_train(const pcr::GuideNet *g, const pcr::GuideRecord *rec, uint32_t rec_cap, int step,
+                                  int S, hipStream_t s);
+hipError_t pcr_launch_guide_adam(const pcr::GuideNet *g, const pcr::GuideTrain *h, int apply, hipStream_t s);
 hipError_t pcr_launch_guide_sync(const pcr::GuideNet *g, hipStream_t s);
 hipError_t pcr_launch_guide_forward(const pcr::GuideNet *g, const float *pn, uint32_t n, float *out, int grid,
                                     hipStream_t s);

@@ -1,186 +1,47 @@
 // guide.hip -- gfx950 kernels of the learned directional proposal: encoding,
-// the 3-layer MLP (forward and backward as 16x16x16 BF16 MFMA tiles with FP32
-// accumulation, one wave per 64 rays or records), vMF-mixture sampling with
-// one-sample MIS against cosine sampling, training-record preparation and Adam.
+// the 3-layer MLP (forward and backward as 16x16x32 BF16 MFMA tiles with FP32
+// accumulation, one wave per 64 rays or records, activations in registers),
+// vMF-mixture sampling with one-sample MIS against cosine sampling,
+// training-record preparation and Adam.
 #include "guide.h"
 #include "path_state.h"
 
 namespace pcr {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// ------------------------------------------------------------ MFMA tiles --
-// One wave computes C[M x N] = A[M x K] * B[K x N] as 16x16x16 tiles of
-// v_mfma_f32_16x16x16_bf16.  Operand layout of the instruction: lane l holds
-// A[l % 16][4 (l / 16) + i], B[4 (l / 16) + i][l % 16] and receives
-// C[4 (l / 16) + i][l % 16], i = 0..3.  a(m, k) and b(k, n) return BF16 bits;
-// out(m, n, v) receives each FP32 result.  Every lane of the wave must be
-// active.
-template <int M, int N, int K, class FA, class FB, class FC>
-__device__ __forceinline__ void wave_gemm(FA a, FB b, FC out) {
-    const int l = (int)__lane_id();
-    const int r = l & 15, q = l >> 4;
-    for (int mt = 0; mt < M / 16; mt++)
-        for (int nt = 0; nt < N / 16; nt++) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int kt = 0; kt < K / 16; kt++) {
-                bf16x4 A, B;
-                _Pragma("unroll") for (int i = 0; i < 4; i++) {
-                    A[i] = (short)a(mt * 16 + r, kt * 16 + 4 * q + i);
-                    B[i] = (short)b(kt * 16 + 4 * q + i, nt * 16 + r);
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc, 0, 0, 0);
-            }
-            _Pragma("unroll") for (int i = 0; i < 4; i++) out(mt * 16 + 4 * q + i, nt * 16 + r, acc[i]);
-        }
-}
 
 // ------------------------------------------------------------- encoding --
 // 6 raw inputs in [0, 1] (position normalised by the scene box, normal mapped
 // from [-1, 1]) followed by sin and cos of 2^j * pi * x, j = 0..7 (in turns:
-// x * 2^(j-1)), padded with zeros to kGIn.
-__device__ __forceinline__ void encode_row(const GuideNet &g, V3 p, V3 n, uint16_t *row) {
-    float s[6] = {fmul(fsub(p.x, g.lo[0]), g.inv[0]), fmul(fsub(p.y, g.lo[1]), g.inv[1]),
-                  fmul(fsub(p.z, g.lo[2]), g.inv[2]), fmul(fadd(n.x, 1.f), 0.5f),
-                  fmul(fadd(n.y, 1.f), 0.5f),       fmul(fadd(n.z, 1.f), 0.5f)};
-    _Pragma("unroll") for (int d = 0; d < 6; d++) row[d] = f32_to_bf16(s[d]);
-    float f = 0.5f;
-    for (int j = 0; j < 8; j++) {
-        _Pragma("unroll") for (int d = 0; d < 6; d++) {
-            float sn, cs;
-            fm_sincos_turns(fmul(s[d], f), sn, cs);
-            row[6 + 12 * j + 2 * d] = f32_to_bf16(sn);
-            row[6 + 12 * j + 2 * d + 1] = f32_to_bf16(cs);
-        }
-        f = fmul(f, 2.f);
+// x * 2^(j-1)), padded with zeros to kGIn.  Each dimension's 8 frequencies are
+// two chains of 4: sin/cos at j = 0 and j = 4 by v_sin_f32 / v_cos_f32 (inputs
+// in turns, reduced exactly to [-1/2, 1/2] first), then three doublings
+// (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a)).  The
+// values stay within 1e-5 of sin/cos evaluated exactly, far below the BF16
+// rounding (2^-9 relative) that follows.
+__device__ __forceinline__ void guide_raw(const GuideNet &g, V3 p, V3 n, float *s) {
+    s[0] = fmul(fsub(p.x, g.lo[0]), g.inv[0]);
+    s[1] = fmul(fsub(p.y, g.lo[1]), g.inv[1]);
+    s[2] = fmul(fsub(p.z, g.lo[2]), g.inv[2]);
+    s[3] = fmul(fadd(n.x, 1.f), 0.5f);
+    s[4] = fmul(fadd(n.y, 1.f), 0.5f);
+    s[5] = fmul(fadd(n.z, 1.f), 0.5f);
+}
+// sin/cos at j = 4 h + t, t = 0..3, of raw input x: v[2 t] = sin, v[2 t + 1] = cos
+__device__ __forceinline__ void freq_chain(float x, int h, float *v) {
+    const float t = fmul(x, h ? 8.f : 0.5f);
+    const float red = fsub(t, rintf(t));
+    float sn = __builtin_amdgcn_sinf(red), cs = __builtin_amdgcn_cosf(red);
+    v[0] = sn;
+    v[1] = cs;
+    _Pragma("unroll") for (int t = 1; t < 4; t++) {
+        const float s2 = fmul(2.f, fmul(sn, cs)), c2 = fmul(fsub(cs, sn), fadd(cs, sn));
+        sn = s2;
+        cs = c2;
+        v[2 * t] = sn;
+        v[2 * t + 1] = cs;
     }
-    for (int k = kGInUsed; k < kGIn; k++) row[k] = 0;
 }
-
-// LDS working set of one wave (64 rows) and the block's copy of the weights.
-// Rows are padded by 4 BF16 so the 16 rows a fragment load touches fall in
-// different banks; every fragment is one 8-byte load (4 consecutive k).
-constexpr int kLdX = kGIn + 4, kLdH = kGHid + 4;
-struct WaveMem {
-    alignas(16) uint16_t x[64 * kLdX];   // encoded inputs (BF16)
-    alignas(16) uint16_t h1[64 * kLdH];  // relu(layer 1) (BF16)
-    alignas(16) uint16_t h2[64 * kLdH];  // relu(layer 2) (BF16)
-    float o[64 * kGOutPad];              // layer 3 outputs (FP32)
-};
-struct WaveGrad {
-    uint16_t dout[64 * kGOutPad];  // dLoss/dOutput (BF16)
-    uint16_t da2[64 * kGHid];      // dLoss/d(pre-activation 2)
-    uint16_t da1[64 * kGHid];      // dLoss/d(pre-activation 1)
-};
-struct NetLds {  // transposed weights [out][in]
-    alignas(16) uint16_t w1t[kGHid * kLdX];
-    alignas(16) uint16_t w2t[kGHid * kLdH];
-    alignas(16) uint16_t w3t[kGOutPad * kLdH];
-};
-
-// Block-cooperative copy of the transposed BF16 weights into LDS.
-__device__ __forceinline__ void load_weights(const GuideNet &g, NetLds &W) {
-    const uint16_t *t1 = g.wb + (kGW1 + kGW2 + kGW3);
-    const uint16_t *t2 = t1 + kGW1, *t3 = t2 + kGW2;
-    for (int i = threadIdx.x; i < kGHid * kGIn / 4; i += blockDim.x) {
-        const int r = (4 * i) / kGIn, c = (4 * i) % kGIn;
-        *(uint2 *)(W.w1t + r * kLdX + c) = *(const uint2 *)(t1 + 4 * i);
-    }
-    for (int i = threadIdx.x; i < kGHid * kGHid / 4; i += blockDim.x) {
-        const int r = (4 * i) / kGHid, c = (4 * i) % kGHid;
-        *(uint2 *)(W.w2t + r * kLdH + c) = *(const uint2 *)(t2 + 4 * i);
-    }
-    for (int i = threadIdx.x; i < kGOutPad * kGHid / 4; i += blockDim.x) {
-        const int r = (4 * i) / kGHid, c = (4 * i) % kGHid;
-        *(uint2 *)(W.w3t + r * kLdH + c) = *(const uint2 *)(t3 + 4 * i);
-    }
-    __syncthreads();
-}
-
-// C[M x N] = A[M x K] * B[K x N] with A row-major in LDS (leading dimension
-// lda) and B given transposed, Bt[n][k] in LDS (ldb): both fragments are
-// single 8-byte LDS loads.
-template <int M, int N, int K, class FC>
-__device__ __forceinline__ void wave_gemm_lds(const uint16_t *A, int lda, const uint16_t *Bt, int ldb, FC out) {
-    const int l = (int)__lane_id();
-    const int r = l & 15, q = l >> 4;
-    for (int mt = 0; mt < M / 16; mt++)
-        for (int nt = 0; nt < N / 16; nt++) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            _Pragma("unroll") for (int kt = 0; kt < K / 16; kt++) {
-                const bf16x4 a = *(const bf16x4 *)(A + (mt * 16 + r) * lda + kt * 16 + 4 * q);
-                const bf16x4 b = *(const bf16x4 *)(Bt + (nt * 16 + r) * ldb + kt * 16 + 4 * q);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-            }
-            _Pragma("unroll") for (int i = 0; i < 4; i++) out(mt * 16 + 4 * q + i, nt * 16 + r, acc[i]);
-        }
-}
-
-// Forward pass of the wave's 64 rows (x already in LDS).
-__device__ __forceinline__ void net_forward(const GuideNet &g, const NetLds &W, WaveMem &M) {
-    const float *b1 = g.w + kGOffB1, *b2 = g.w + kGOffB2, *b3 = g.w + kGOffB3;
-    wave_gemm_lds<64, kGHid, kGIn>(M.x, kLdX, W.w1t, kLdX, [&](int m, int n, float v) {
-        M.h1[m * kLdH + n] = f32_to_bf16(smax(0.f, fadd(v, b1[n])));
-    });
-    __syncthreads();
-    wave_gemm_lds<64, kGHid, kGHid>(M.h1, kLdH, W.w2t, kLdH, [&](int m, int n, float v) {
-        M.h2[m * kLdH + n] = f32_to_bf16(smax(0.f, fadd(v, b2[n])));
-    });
-    __syncthreads();
-    wave_gemm_lds<64, kGOutPad, kGHid>(M.h2, kLdH, W.w3t, kLdH, [&](int m, int n, float v) {
-        M.o[m * kGOutPad + n] = fadd(v, b3[n]);
-    });
-    __syncthreads();
-}
-
-// ------------------------------------------------------------ mixture ----
-struct Lobe {
-    V3 mu;
-    float kappa, e2, len;  // e2 = exp(-2 kappa); len = |unnormalised mean|
-};
-__device__ __forceinline__ Lobe lobe_of(const float *o, int k) {
-    Lobe L;
-    V3 m = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
-    L.len = sqrtf(dot(m, m));
-    L.mu = L.len > 1e-12f ? mul(m, 1.f / L.len) : v3(0.f, 0.f, 1.f);
-    L.kappa = expf(fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax));
-    L.e2 = expf(-2.f * L.kappa);
-    return L;
-}
-// log of the vMF density on the sphere: kappa / (2 pi (1 - e^-2k)) e^{k (mu.w - 1)}
-__device__ __forceinline__ float vmf_log_pdf(const Lobe &L, V3 w) {
-    return logf(L.kappa) - 1.8378770664093453f - logf(1.f - L.e2) + L.kappa * (dot(L.mu, w) - 1.f);
-}
-// weights: softmax of the logits
-__device__ __forceinline__ void mix_weights(const float *o, float *pi) {
-    float amax = o[4];
-    for (int k = 1; k < kGLobes; k++) amax = fmaxf(amax, o[5 * k + 4]);
-    float s = 0.f;
-    for (int k = 0; k < kGLobes; k++) {
-        pi[k] = expf(o[5 * k + 4] - amax);
-        s += pi[k];
-    }
-    for (int k = 0; k < kGLobes; k++) pi[k] /= s;
-}
-__device__ __forceinline__ float mix_pdf(const float *o, const float *pi, V3 w) {
-    float p = 0.f;
-    for (int k = 0; k < kGLobes; k++) p += pi[k] * expf(vmf_log_pdf(lobe_of(o, k), w));
-    return p;
-}
-// vMF sample about mu (Wood 1994): cos(theta) = 1 + log(u1 + (1 - u1) e^-2k) / k
-__device__ __forceinline__ V3 vmf_sample(const Lobe &L, float u1, float u2) {
-    float c = 1.f + logf(u1 + (1.f - u1) * L.e2) / L.kappa;
-    c = fminf(1.f, fmaxf(-1.f, c));
-    float s = sqrtf(fmaxf(0.f, 1.f - c * c));
-    V3 helper = fabsf(L.mu.x) <= fabsf(L.mu.y) ? v3(1.f, 0.f, 0.f) : v3(0.f, 1.f, 0.f);
-    V3 T = normalize(cross(L.mu, helper));
-    V3 B = cross(L.mu, T);
-    float sn, cs;
-    fm_sincos_turns(u2, sn, cs);
-    return add(add(mul(T, s * cs), mul(B, s * sn)), mul(L.mu, c));
-}
-
 __device__ __forceinline__ uint32_t record_hash(uint32_t path, uint32_t bounce) {
     uint32_t h = path * 0x9E3779B1u ^ (bounce + 1u) * 0x85EBCA77u;
     h ^= h >> 15;
@@ -189,87 +50,343 @@ __device__ __forceinline__ uint32_t record_hash(uint32_t path, uint32_t bounce) 
     return h >> 8;  // 24 bits
 }
 
+// ------------------------------------------------ register-resident MLP --
+typedef __bf16 bh8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bh4 __attribute__((ext_vector_type(4)));
+
+constexpr int kORow = 41;  // FP32 output row of one ray in LDS (odd: conflict-free row reads)
+constexpr int kRRow = 9;   // raw-input row of one ray in LDS
+
+// Weights of the block in LDS: the MFMA fragments and the FP32 biases.
+struct FragLds {
+    alignas(16) uint16_t f[kGFragElems];
+    alignas(16) float b1[kGHid];
+    alignas(16) float b2[kGHid];
+    alignas(16) float b3[kGOutPad];
+};
+// Per wave: raw inputs of its 64 rays (read by layer 1), then the 40 outputs
+// of each ray (written by layer 3) in the same words.
+constexpr int kWaveWords = 64 * kORow;
+constexpr int kRawOff = kWaveWords - 64 * kRRow;
+static_assert(kRawOff >= 32 * kORow, "raw rows lie past the first half's output rows");
+
+__device__ __forceinline__ void load_frags(const GuideNet &g, FragLds &W) {
+    const uint4 *src = (const uint4 *)g.wf;
+    uint4 *dst = (uint4 *)W.f;
+    for (int i = threadIdx.x; i < kGFragElems / 8; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kGHid; i += blockDim.x) {
+        W.b1[i] = g.w[kGOffB1 + i];
+        W.b2[i] = g.w[kGOffB2 + i];
+    }
+    for (int i = threadIdx.x; i < kGOutPad; i += blockDim.x) W.b3[i] = g.w[kGOffB3 + i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ bh8 frag(const FragLds &W, int f, int lane) {
+    return *(const bh8 *)(W.f + (f * 64 + lane) * 8);
+}
+
+// Layer-1 B operand of K-block kb for the ray whose raw inputs are s (in LDS).
+__device__ __forceinline__ bh8 encode_slots(const float *s, int kb, int q) {
+    bh8 B;
+    if (kb < 3) {
+        const int c = 4 * kb + q;
+        float v[8];
+        freq_chain(s[c % 6], c / 6, v);
+        _Pragma("unroll") for (int i = 0; i < 8; i++) B[i] = (__bf16)v[i];
+    } else {
+        const bool raw = q == 0;
+        _Pragma("unroll") for (int i = 0; i < 6; i++) B[i] = (__bf16)(raw ? s[i] : 0.f);
+        B[6] = (__bf16)0.f;
+        B[7] = (__bf16)0.f;
+    }
+    return B;
+}
+
+// bias, ReLU and BF16 rounding of the lane's 4 values of one output tile
+__device__ __forceinline__ bh4 act(f32x4 acc, const float *b) {
+    const float4 bb = *(const float4 *)b;
+    bh4 h;
+    h[0] = (__bf16)smax(0.f, fadd(acc[0], bb.x));
+    h[1] = (__bf16)smax(0.f, fadd(acc[1], bb.y));
+    h[2] = (__bf16)smax(0.f, fadd(acc[2], bb.z));
+    h[3] = (__bf16)smax(0.f, fadd(acc[3], bb.w));
+    return h;
+}
+
+// Forward pass of one wave's 64 rays.  On entry io[kRawOff + ray * kRRow + d]
+// holds the raw inputs; on exit io[ray * kORow + k], k < kGOut, holds the
+// outputs.  Two halves of 32 rays (2 ray tiles) keep the live accumulators
+// and activations at 48 registers; each half reads the weight fragments once.
+// The raw rows sit at the end of the region, past the first half's output
+// rows, and the second half overwrites them only after reading them.  Every
+// lane of the wave must be active.
+__device__ __forceinline__ void mlp_half(const FragLds &W, const float *io, int half, f32x4 (&acc)[2][4]) {
+    const int lane = (int)__lane_id(), r = lane & 15, q = lane >> 4;
+    _Pragma("unroll") for (int j = 0; j < 2; j++)
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) acc[j][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // layer 1: 4 K-blocks x 4 output tiles x 2 ray tiles
+    _Pragma("unroll") for (int kb = 0; kb < 4; kb++) {
+        bh8 A[4];
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) A[mt] = frag(W, 4 * mt + kb, lane);
+        _Pragma("unroll") for (int j = 0; j < 2; j++) {
+            const bh8 B = encode_slots(io + ((2 * half + j) * 16 + r) * kRRow, kb, q);
+            _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+                acc[j][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B, acc[j][mt], 0, 0, 0);
+        }
+    }
+    bh4 h[2][4];
+    _Pragma("unroll") for (int j = 0; j < 2; j++)
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) {
+            h[j][mt] = act(acc[j][mt], W.b1 + mt * 16 + 4 * q);
+            acc[j][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    // layer 2
+    _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+        bh8 A[4];
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) A[mt] = frag(W, 16 + 2 * mt + kb, lane);
+        _Pragma("unroll") for (int j = 0; j < 2; j++) {
+            const bh8 B = __builtin_shufflevector(h[j][2 * kb], h[j][2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+            _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+                acc[j][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B, acc[j][mt], 0, 0, 0);
+        }
+    }
+    _Pragma("unroll") for (int j = 0; j < 2; j++)
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) {
+            h[j][mt] = act(acc[j][mt], W.b2 + mt * 16 + 4 * q);
+            acc[j][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    // layer 3: 3 output tiles (40 outputs used of 48), bias added
+    _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+        bh8 A[3];
+        _Pragma("unroll") for (int mt = 0; mt < 3; mt++) A[mt] = frag(W, 24 + 2 * mt + kb, lane);
+        _Pragma("unroll") for (int j = 0; j < 2; j++) {
+            const bh8 B = __builtin_shufflevector(h[j][2 * kb], h[j][2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+            _Pragma("unroll") for (int mt = 0; mt < 3; mt++)
+                acc[j][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B, acc[j][mt], 0, 0, 0);
+        }
+    }
+    _Pragma("unroll") for (int j = 0; j < 2; j++)
+        _Pragma("unroll") for (int mt = 0; mt < 3; mt++) {
+            const float4 bb = *(const float4 *)(W.b3 + mt * 16 + 4 * q);
+            acc[j][mt][0] = fadd(acc[j][mt][0], bb.x);
+            acc[j][mt][1] = fadd(acc[j][mt][1], bb.y);
+            acc[j][mt][2] = fadd(acc[j][mt][2], bb.z);
+            acc[j][mt][3] = fadd(acc[j][mt][3], bb.w);
+        }
+}
+__device__ __forceinline__ void put_outputs(float *io, int half, const f32x4 (&acc)[2][4]) {
+    const int lane = (int)__lane_id(), r = lane & 15, q = lane >> 4;
+    _Pragma("unroll") for (int j = 0; j < 2; j++)
+        _Pragma("unroll") for (int mt = 0; mt < 3; mt++)
+            _Pragma("unroll") for (int i = 0; i < 4; i++) {
+                const int m = mt * 16 + 4 * q + i;
+                if (m < kGOut) io[((2 * half + j) * 16 + r) * kORow + m] = acc[j][mt][i];
+            }
+}
+__device__ __forceinline__ void mlp_wave(const FragLds &W, float *io) {
+    _Pragma("unroll 1") for (int half = 0; half < 2; half++) {
+        f32x4 acc[2][4];
+        mlp_half(W, io + kRawOff, half, acc);
+        __builtin_amdgcn_wave_barrier();  // this half's raw-input reads precede its output writes
+        put_outputs(io, half, acc);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// ------------------------------------------------------------ mixture ----
+// The lane's mixture in sampling form: unit means, concentrations, and per
+// lobe log(pi_k) + log(kappa / (2 pi (1 - e^-2k))), so that
+// q(w) = sum_k exp(lw_k + kappa_k (mu_k . w - 1)).  Hardware exp/log/rcp/sqrt:
+// the sampler and the density it reports use the same values.
+struct Mix {
+    float mx[kGLobes], my[kGLobes], mz[kGLobes], kap[kGLobes], lw[kGLobes], e[kGLobes];
+    float esum;
+};
+__device__ __forceinline__ bool mix_from(const float *o, Mix &X) {
+    bool ok = true;
+    float amax = o[4];
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) {
+        _Pragma("unroll") for (int c = 0; c < 5; c++) ok = ok && isfinite(o[5 * k + c]);
+        amax = fmaxf(amax, o[5 * k + 4]);
+    }
+    X.esum = 0.f;
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) {
+        X.e[k] = __expf(o[5 * k + 4] - amax);
+        X.esum += X.e[k];
+    }
+    const float lsum = __logf(X.esum);
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) {
+        const float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
+        const float l2 = x * x + y * y + z * z;
+        const bool live = l2 > 1e-24f;
+        const float il = live ? __builtin_amdgcn_rsqf(l2) : 0.f;
+        X.mx[k] = live ? x * il : 0.f;
+        X.my[k] = live ? y * il : 0.f;
+        X.mz[k] = live ? z * il : 1.f;
+        const float rk = fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax);
+        X.kap[k] = __expf(rk);
+        const float e2 = __expf(-2.f * X.kap[k]);
+        X.lw[k] = (o[5 * k + 4] - amax) - lsum + rk - 1.8378770664093453f - __logf(1.f - e2);
+    }
+    return ok;
+}
+__device__ __forceinline__ float mix_density(const Mix &X, V3 w) {
+    float p = 0.f;
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++)
+        p += __expf(X.lw[k] + X.kap[k] * (X.mx[k] * w.x + X.my[k] * w.y + X.mz[k] * w.z - 1.f));
+    return p;
+}
+// The lobe picked by u * (sum of the unnormalised weights) and the cosine of a
+// vMF sample about its mean (Wood 1994): cos(theta) = 1 + log(u1 + (1 - u1)
+// e^-2k) / k.  Selects only, so the wave does not diverge.
+__device__ __forceinline__ void mix_pick(const Mix &X, float u_lobe, float u1, V3 &mu, float &c) {
+    const float target = u_lobe * X.esum;
+    float cum = 0.f, mx = X.mx[kGLobes - 1], my = X.my[kGLobes - 1], mz = X.mz[kGLobes - 1],
+          kap = X.kap[kGLobes - 1];
+    bool found = false;
+    _Pragma("unroll") for (int k = 0; k < kGLobes - 1; k++) {
+        cum += X.e[k];
+        const bool take = !found && target < cum;
+        mx = take ? X.mx[k] : mx;
+        my = take ? X.my[k] : my;
+        mz = take ? X.mz[k] : mz;
+        kap = take ? X.kap[k] : kap;
+        found = found || take;
+    }
+    mu = v3(mx, my, mz);
+    const float e2 = __expf(-2.f * kap);
+    c = fminf(1.f, fmaxf(-1.f, 1.f + __logf(u1 + (1.f - u1) * e2) * __builtin_amdgcn_rcpf(kap)));
+}
+// Direction at polar cosine c and azimuth 2 pi u2 about the unit axis, in the
+// frame T = normalize(axis x h), B = axis x T (h = x or y axis, whichever is
+// further from the axis, as in Ray.h:28-53).  With axis = N and c = sqrt(1 -
+// u1) this is cosine-weighted hemisphere sampling.
+__device__ __forceinline__ V3 about_axis(V3 axis, float c, float u2) {
+    const float s = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - c * c));
+    const V3 helper = fabsf(axis.x) <= fabsf(axis.y) ? v3(1.f, 0.f, 0.f) : v3(0.f, 1.f, 0.f);
+    V3 T = cross(axis, helper);
+    T = mul(T, __builtin_amdgcn_rsqf(dot(T, T)));
+    const V3 B = cross(axis, T);
+    const float red = u2 - rintf(u2);
+    const float sn = __builtin_amdgcn_sinf(red), cs = __builtin_amdgcn_cosf(red);
+    return add(add(mul(T, s * cs), mul(B, s * sn)), mul(axis, c));
+}
+
 // --------------------------------------------------------------- k_guide --
-// Draws the pending directions of the queue: the MLP runs on 64 rays per wave
+// Draws the pending directions of the queue.  Persistent blocks of kGuideWaves
+// waves share one LDS copy of the weights; each wave takes 64 rays at a time
 // (rays without a pending direction ride along and are left untouched).
 // Draw order from the path's stream: select, lobe, u1, u2.
-__global__ __launch_bounds__(64) void k_guide(GuideArgs a) {
-    __shared__ WaveMem M;
-    __shared__ NetLds W;
+constexpr int kGuideWaves = 12;
+struct GuideBlockLds {
+    FragLds W;
+    float io[kGuideWaves][kWaveWords];
+};
+
+__global__ __launch_bounds__(64 * kGuideWaves) void k_guide(GuideArgs a) {
+    __shared__ GuideBlockLds L;
     const uint32_t n = min(*a.count, a.cap);
-    if (blockIdx.x * 64u >= n) return;
-    load_weights(a.net, W);
-    const int lane = (int)__lane_id();
-    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    if (blockIdx.x * (64u * kGuideWaves) >= n) return;  // uniform per block
+    load_frags(a.net, L.W);
+    const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6);
+    float *io = L.io[wv];
+    for (uint32_t base = (blockIdx.x * kGuideWaves + wv) * 64u; base < n;
+         base += gridDim.x * (64u * kGuideWaves)) {
         const uint32_t i = base + (uint32_t)lane;
-        PathState ps;
-        bool pend = false;
-        if (i < n) {
-            st_load(a.q, i, ps);
-            pend = (ps.meta & ST_GUIDE) != 0;
-        }
+        float4 od1 = make_float4(0.f, 0.f, 1.f, 0.f), od0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < n) od1 = ldg(a.q.od1 + i);
+        uint32_t meta = __float_as_uint(od1.w);
+        const bool pend = i < n && (meta & ST_GUIDE) != 0;
         if (!__ballot(pend)) continue;  // uniform: the whole wave skips
-        encode_row(a.net, pend ? ps.o : v3(0.f, 0.f, 0.f), pend ? ps.d : v3(0.f, 0.f, 1.f), M.x + lane * kLdX);
-        __syncthreads();
-        net_forward(a.net, W, M);
+        float4 tp = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint4 rg = make_uint4(0u, 0u, 0u, 0u);
         if (pend) {
-            const float *o = M.o + lane * kGOutPad;
-            bool net_ok = true;
-            for (int k = 0; k < kGOut; k++) net_ok = net_ok && isfinite(o[k]);
-            float pi[kGLobes];
-            mix_weights(o, pi);
-            const V3 N = ps.d;
-            const float u_sel = rng_next(ps.rng), u_lobe = rng_next(ps.rng);
-            const float u1 = rng_next(ps.rng), u2 = rng_next(ps.rng);
-            V3 w;
-            float pdf;
-            if (!net_ok) {  // fallback: plain cosine sampling
-                w = fm_hemisphere(N, u1, u2);
-                pdf = fmaxf(dot(N, w), 0.f) * 0.31830988618379067f;
-            } else {
-                if (u_sel < 0.5f) {
-                    w = fm_hemisphere(N, u1, u2);
-                } else {
-                    int k = 0;
-                    float cum = pi[0];
-                    while (k < kGLobes - 1 && u_lobe >= cum) cum += pi[++k];
-                    w = vmf_sample(lobe_of(o, k), u1, u2);
-                }
-                pdf = 0.5f * fmaxf(dot(N, w), 0.f) * 0.31830988618379067f + 0.5f * mix_pdf(o, pi, w);
+            od0 = ldg(a.q.od0 + i);
+            tp = ldg(a.q.tp + i);
+            rg = a.q.rng[i];
+        }
+        const V3 N = pend ? v3(od1.x, od1.y, od1.z) : v3(0.f, 0.f, 1.f);
+        {
+            float s[6];
+            guide_raw(a.net, v3(od0.x, od0.y, od0.z), N, s);
+            _Pragma("unroll") for (int d = 0; d < 6; d++) io[kRawOff + lane * kRRow + d] = s[d];
+        }
+        if (a.mix) {
+            __builtin_amdgcn_wave_barrier();
+            mlp_wave(L.W, io);
+        }
+        bool rec = false;
+        V3 w = N;
+        float pdf = 0.f;
+        if (pend) {
+            Mix X;
+            bool net_ok = false;
+            if (a.mix) {
+                float o[kGOut];
+                _Pragma("unroll") for (int k = 0; k < kGOut; k++) o[k] = io[lane * kORow + k];
+                net_ok = mix_from(o, X);
             }
+            uint64_t st = (uint64_t)rg.x | ((uint64_t)rg.y << 32);
+            const float u_sel = rng_next(st), u_lobe = rng_next(st);
+            const float u1 = rng_next(st), u2 = rng_next(st);
+            rg.x = (uint32_t)st;
+            rg.y = (uint32_t)(st >> 32);
+            // cosine half (and the fallback): axis N, cos(theta) = sqrt(1 - u1)
+            V3 axis = N;
+            float c = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - u1));
+            const bool vmf = net_ok && u_sel >= 0.5f;
+            if (net_ok) {
+                V3 mu;
+                float cv;
+                mix_pick(X, u_lobe, u1, mu, cv);
+                axis = vmf ? mu : axis;
+                c = vmf ? cv : c;
+            }
+            w = about_axis(axis, c, u2);
             const float cosw = dot(N, w);
-            const int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
-            ps.meta &= ~ST_GUIDE;
+            pdf = net_ok ? 0.5f * fmaxf(cosw, 0.f) * 0.31830988618379067f + 0.5f * mix_density(X, w)
+                         : fmaxf(cosw, 0.f) * 0.31830988618379067f;
+            const int nch = st_nch(meta);
+            meta &= ~ST_GUIDE;
             if (cosw <= 0.f || !(pdf > 0.f)) {
-                _Pragma("unroll") for (int c = 0; c < 4; c++) ps.tp[c] = 0.f;
-                ps.meta |= ST_DEAD;
+                tp = make_float4(0.f, 0.f, 0.f, 0.f);
+                meta |= ST_DEAD;
             } else {
                 // cosine BSDF sample weight (the albedo is already applied):
                 // (cos / pi) / pdf, which is 1 for pure cosine sampling
                 const float r = cosw * 0.31830988618379067f / pdf;
-                _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.tp[c] *= r;
-                if (record_hash(ps.path, (uint32_t)st_bounce(ps.meta)) < a.keep) {
-                    uint32_t slot = atomicAdd(a.net.n_rec, 1u);
-                    if (slot < a.rec_cap) {
-                        GuideRecord &R = a.rec[slot];
-                        R.o[0] = ps.o.x; R.o[1] = ps.o.y; R.o[2] = ps.o.z;
-                        R.n[0] = N.x; R.n[1] = N.y; R.n[2] = N.z;
-                        R.w[0] = w.x; R.w[1] = w.y; R.w[2] = w.z;
-                        R.pdf = pdf;
-                        R.path = ps.path;
-                        R.chan = (uint32_t)ch0 | ((uint32_t)nch << 8);
-                        _Pragma("unroll") for (int c = 0; c < 4; c++) {
-                            R.rad[c] = ps.rad[c];
-                            R.tp[c] = ps.tp[c];
-                        }
-                        R.weight = 0.f;
-                    }
-                }
+                if (nch > 0) tp.x *= r;
+                if (nch > 1) tp.y *= r;
+                if (nch > 2) tp.z *= r;
+                if (nch > 3) tp.w *= r;
+                rec = record_hash(rg.z, (uint32_t)st_bounce(meta)) < a.keep;
             }
-            ps.d = w;
-            st_store(a.q, i, ps);
+            stg(a.q.od1 + i, make_float4(w.x, w.y, w.z, __uint_as_float(meta)));
+            stg(a.q.tp + i, tp);
+            a.q.rng[i] = rg;
         }
-        __syncthreads();
+        // training records: one counter update per wave
+        const uint64_t rb = __ballot(rec);
+        if (rb) {
+            uint32_t first = 0;
+            if (lane == __ffsll((unsigned long long)rb) - 1) first = atomicAdd(a.net.n_rec, (uint32_t)__popcll(rb));
+            first = __shfl(first, __ffsll((unsigned long long)rb) - 1, 64);
+            const uint32_t slot = first + (uint32_t)__popcll(rb & ((1ull << lane) - 1ull));
+            if (rec && slot < a.rec_cap) {
+                const float4 rd = ldg(a.q.rad + i);
+                GuideRecord &R = a.rec[slot];
+                R.o[0] = od0.x; R.o[1] = od0.y; R.o[2] = od0.z;
+                R.n[0] = N.x; R.n[1] = N.y; R.n[2] = N.z;
+                R.w[0] = w.x; R.w[1] = w.y; R.w[2] = w.z;
+                R.pdf = pdf;
+                R.path = rg.z;
+                R.chan = (uint32_t)st_ch0(meta) | ((uint32_t)st_nch(meta) << 8);
+                R.rad[0] = rd.x; R.rad[1] = rd.y; R.rad[2] = rd.z; R.rad[3] = rd.w;
+                R.tp[0] = tp.x; R.tp[1] = tp.y; R.tp[2] = tp.z; R.tp[3] = tp.w;
+                R.weight = 0.f;
+            }
+        }
     }
 }
 
@@ -318,147 +435,281 @@ __global__ void k_guide_prep(GuideNet g, GuideRecord *rec, uint32_t rec_cap, con
 }
 
 // dLoss/dOutput of one record, loss = -scale * log q(w) with q the learned vMF
-// mixture; returns the record's loss.
+// mixture; returns the record's loss.  Hardware exp/log/rcp/sqrt (relative
+// error ~1e-6, far inside what BF16 training resolves).
 __device__ __forceinline__ float output_grad(const float *o, V3 w, float scale, float *d) {
-    float pi[kGLobes], t[kGLobes];
-    mix_weights(o, pi);
-    Lobe L[kGLobes];
+    float amax = o[4];
+    _Pragma("unroll") for (int k = 1; k < kGLobes; k++) amax = fmaxf(amax, o[5 * k + 4]);
+    float e[kGLobes], esum = 0.f;
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) {
+        e[k] = __expf(o[5 * k + 4] - amax);
+        esum += e[k];
+    }
+    const float ies = __builtin_amdgcn_rcpf(esum), lsum = __logf(esum);
+    float t[kGLobes], kap[kGLobes], e2[kGLobes], len[kGLobes], c[kGLobes];
+    V3 mu[kGLobes];
     float tmax = -3.0e38f;
-    for (int k = 0; k < kGLobes; k++) {
-        L[k] = lobe_of(o, k);
-        t[k] = logf(fmaxf(pi[k], 1e-30f)) + vmf_log_pdf(L[k], w);
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) {
+        const V3 m = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+        len[k] = __builtin_amdgcn_sqrtf(dot(m, m));
+        mu[k] = len[k] > 1e-12f ? mul(m, __builtin_amdgcn_rcpf(len[k])) : v3(0.f, 0.f, 1.f);
+        const float rk = fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax);
+        kap[k] = __expf(rk);
+        e2[k] = __expf(-2.f * kap[k]);
+        c[k] = dot(mu[k], w);
+        // log pi_k + log vMF_k(w)
+        t[k] = (o[5 * k + 4] - amax - lsum) + rk - 1.8378770664093453f - __logf(1.f - e2[k]) + kap[k] * (c[k] - 1.f);
         tmax = fmaxf(tmax, t[k]);
     }
-    float s = 0.f;
-    for (int k = 0; k < kGLobes; k++) s += expf(t[k] - tmax);
-    const float logq = tmax + logf(s);
-    for (int k = 0; k < kGLobes; k++) {
-        const float gam = expf(t[k] - logq);
-        const float c = dot(L[k].mu, w);
+    float sum = 0.f;
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) sum += __expf(t[k] - tmax);
+    const float logq = tmax + __logf(sum);
+    _Pragma("unroll") for (int k = 0; k < kGLobes; k++) {
+        const float gam = __expf(t[k] - logq);
         const float rk = o[5 * k + 3];
         const bool live = rk > kGLogKappaMin && rk < kGLogKappaMax;
-        const float kap = L[k].kappa;
-        const float dr = live ? gam * kap * (1.f / kap - 2.f * L[k].e2 / (1.f - L[k].e2) + (c - 1.f)) : 0.f;
-        const float dm = L[k].len > 1e-12f ? gam * kap / L[k].len : 0.f;
-        d[5 * k + 0] = -scale * dm * (w.x - c * L[k].mu.x);
-        d[5 * k + 1] = -scale * dm * (w.y - c * L[k].mu.y);
-        d[5 * k + 2] = -scale * dm * (w.z - c * L[k].mu.z);
+        const float dr = live ? gam * kap[k] *
+                                    (__builtin_amdgcn_rcpf(kap[k]) -
+                                     2.f * e2[k] * __builtin_amdgcn_rcpf(1.f - e2[k]) + (c[k] - 1.f))
+                              : 0.f;
+        const float dm = len[k] > 1e-12f ? gam * kap[k] * __builtin_amdgcn_rcpf(len[k]) : 0.f;
+        d[5 * k + 0] = -scale * dm * (w.x - c[k] * mu[k].x);
+        d[5 * k + 1] = -scale * dm * (w.y - c[k] * mu[k].y);
+        d[5 * k + 2] = -scale * dm * (w.z - c[k] * mu[k].z);
         d[5 * k + 3] = -scale * dr;
-        d[5 * k + 4] = -scale * (gam - pi[k]);
+        d[5 * k + 4] = -scale * (gam - e[k] * ies);
     }
-    for (int k = kGOut; k < kGOutPad; k++) d[k] = 0.f;
+    _Pragma("unroll") for (int k = kGOut; k < kGOutPad; k++) d[k] = 0.f;
     return -scale * logq;
 }
 
+// A forward fragment read straight from global memory (training reads each
+// fragment once per tile, so staging it in LDS would not save a load).
+__device__ __forceinline__ bh8 gfrag(const uint16_t *base, int f, int lane) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    return __builtin_bit_cast(bh8, *(const PCR_GLOBAL u32x4 *)(const void *)(base + (f * 64 + lane) * 8));
+}
+__device__ __forceinline__ float4 bias4(const float *b) { return ldg((const float4 *)b); }
+__device__ __forceinline__ bh4 act4(f32x4 acc, float4 bb) {
+    bh4 h;
+    h[0] = (__bf16)smax(0.f, fadd(acc[0], bb.x));
+    h[1] = (__bf16)smax(0.f, fadd(acc[1], bb.y));
+    h[2] = (__bf16)smax(0.f, fadd(acc[2], bb.z));
+    h[3] = (__bf16)smax(0.f, fadd(acc[3], bb.w));
+    return h;
+}
+__device__ __forceinline__ uint16_t bits(__bf16 h) { return __builtin_bit_cast(uint16_t, h); }
+__device__ __forceinline__ float bf(uint16_t u) { return bf16_to_f32(u); }
+
+// LDS of one training wave.  Transposed copies ([row][ray], row stride kTS
+// BF16) feed the weight-gradient products, whose K dimension is the 64 rays.
+constexpr int kTS = 64 + 8;
+struct TrainLds {
+    float raw[64 * kRRow];                // raw inputs per ray
+    float io[64 * kORow];                 // outputs per ray
+    alignas(16) uint16_t xt[128 * kTS];   // layer-1 inputs, [K slot][ray]
+    alignas(16) uint16_t h1t[64 * kTS];   // relu(layer 1), [unit][ray]
+    alignas(16) uint16_t h2t[64 * kTS];   // relu(layer 2), [unit][ray]
+    alignas(16) uint16_t dor[64 * kTS];   // dLoss/dOutput, [ray][output] (outputs 48..63 zero)
+    alignas(16) uint16_t dot[48 * kTS];   // dLoss/dOutput, [output][ray]
+    alignas(16) uint16_t da2t[64 * kTS];  // dLoss/d(pre-activation 2), [unit][ray]
+    alignas(16) uint16_t da1t[64 * kTS];  // dLoss/d(pre-activation 1), [unit][ray]
+    float acc[kGParams];                  // the block's gradient sums
+};
+
 // One training step's gradient: records [s n / S, (s + 1) n / S), each
-// weighted by weight / wsum[s].  Forward and backward as MFMA tiles; weight
-// gradients are added to g.grad.
-__global__ __launch_bounds__(64) void k_guide_grad(GuideNet g, const GuideRecord *rec, uint32_t rec_cap, int s,
-                                                   int S) {
-    __shared__ WaveMem M;
-    __shared__ WaveGrad D;
-    __shared__ NetLds Wl;
+// weighted by weight / wsum[s], in 64-record tiles over kGTrainBlocks blocks
+// of kTrainWaves waves (a block without a tile writes a zero row).  Wave w takes rays 16 w .. 16 w + 15 of the tile
+// through the forward and backward passes (MFMA tiles, activations in
+// registers, one 16-ray column block each); the weight-gradient products (K =
+// the tile's 64 rays, from the transposed copies in LDS) are then split over
+// the waves.  Each block writes its gradient sums to its row of g.part (the
+// Adam kernel adds the rows) and adds its loss to wsum[2 s + 1].
+constexpr int kTrainWaves = 4;
+__global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train(GuideNet g, const GuideRecord *rec,
+                                                                  uint32_t rec_cap, int s, int S) {
+    __shared__ TrainLds L;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wv = tid >> 6, r = lane & 15, q = lane >> 4;
+    const int ray = wv * 16 + r;  // the tile ray of this lane's B-operand column
     const uint32_t n = min(*g.n_rec, rec_cap);
     const uint32_t r0 = (uint32_t)((uint64_t)s * n / S), r1 = (uint32_t)((uint64_t)(s + 1) * n / S);
     const float W = g.wsum[2 * s];
-    if (!(W > 0.f) || r0 + blockIdx.x * 64u >= r1) return;
-    load_weights(g, Wl);
-    const int lane = (int)__lane_id();
-    const uint16_t *w2 = g.wb + kGW1, *w3 = w2 + kGW2;  // [in][out] layouts
-    float *gw1 = g.grad + kGOffW1, *gb1 = g.grad + kGOffB1, *gw2 = g.grad + kGOffW2, *gb2 = g.grad + kGOffB2;
-    float *gw3 = g.grad + kGOffW3, *gb3 = g.grad + kGOffB3;
-    for (uint32_t base = r0 + blockIdx.x * 64u; base < r1; base += gridDim.x * 64u) {
-        const uint32_t r = base + (uint32_t)lane;
-        const bool ok = r < r1 && rec[r].weight > 0.f;
-        GuideRecord R = rec[ok ? r : r0];
-        encode_row(g, v3(R.o[0], R.o[1], R.o[2]), v3(R.n[0], R.n[1], R.n[2]), M.x + lane * kLdX);
-        __syncthreads();
-        net_forward(g, Wl, M);
-        float d[kGOutPad];
-        if (ok) {
-            const float loss = output_grad(M.o + lane * kGOutPad, v3(R.w[0], R.w[1], R.w[2]), R.weight / W, d);
-            atomicAdd(g.wsum + 2 * s + 1, loss);
-        } else {
-            for (int k = 0; k < kGOutPad; k++) d[k] = 0.f;
+    for (int p = tid; p < kGParams; p += blockDim.x) L.acc[p] = 0.f;
+    float loss = 0.f;
+    for (uint32_t t0 = r0 + blockIdx.x * 64u; W > 0.f && t0 < r1; t0 += gridDim.x * 64u) {
+        __syncthreads();  // the previous tile's readers of L are done; L.acc is cleared
+        // the lanes of quarter 0 own the rays: record, raw inputs, output gradient
+        const uint32_t ri = t0 + (uint32_t)ray;
+        float scale = 0.f;
+        V3 wdir = v3(0.f, 0.f, 1.f);
+        if (q == 0) {
+            const GuideRecord *R = rec + (ri < r1 ? ri : r0);
+            const float4 a0 = ldg((const float4 *)R), a1 = ldg((const float4 *)R + 1), a5 = ldg((const float4 *)R + 5);
+            // a0 = o.xyz, n.x; a1 = n.yz, w.xy; word 8 = w.z; word 20 = weight
+            wdir = v3(a1.z, a1.w, ldg((const float *)R + 8));
+            scale = (ri < r1 && a5.x > 0.f) ? a5.x / W : 0.f;
+            float sr[6];
+            guide_raw(g, v3(a0.x, a0.y, a0.z), v3(a0.w, a1.x, a1.y), sr);
+            _Pragma("unroll") for (int d = 0; d < 6; d++) L.raw[ray * kRRow + d] = sr[d];
         }
-        for (int k = 0; k < kGOutPad; k++) D.dout[lane * kGOutPad + k] = f32_to_bf16(d[k]);
-        __syncthreads();
-        // dA2 = (dOut W3^T) masked by relu(h2); W3 is [hid][outpad]
-        wave_gemm<64, kGHid, kGOutPad>([&](int m, int k) { return D.dout[m * kGOutPad + k]; },
-                                       [&](int k, int nn) { return w3[nn * kGOutPad + k]; },
-                                       [&](int m, int nn, float v) {
-                                           D.da2[m * kGHid + nn] = M.h2[m * kLdH + nn] ? f32_to_bf16(v) : (uint16_t)0;
-                                       });
-        // dW3 = h2^T dOut
-        wave_gemm<kGHid, kGOutPad, 64>([&](int m, int k) { return M.h2[k * kLdH + m]; },
-                                       [&](int k, int nn) { return D.dout[k * kGOutPad + nn]; },
-                                       [&](int m, int nn, float v) { if (v != 0.f) atomicAdd(gw3 + m * kGOutPad + nn, v); });
-        __syncthreads();
-        // dA1 = (dA2 W2^T) masked by relu(h1); W2 is [hid][hid]
-        wave_gemm<64, kGHid, kGHid>([&](int m, int k) { return D.da2[m * kGHid + k]; },
-                                    [&](int k, int nn) { return w2[nn * kGHid + k]; },
-                                    [&](int m, int nn, float v) {
-                                        D.da1[m * kGHid + nn] = M.h1[m * kLdH + nn] ? f32_to_bf16(v) : (uint16_t)0;
-                                    });
-        // dW2 = h1^T dA2
-        wave_gemm<kGHid, kGHid, 64>([&](int m, int k) { return M.h1[k * kLdH + m]; },
-                                    [&](int k, int nn) { return D.da2[k * kGHid + nn]; },
-                                    [&](int m, int nn, float v) { if (v != 0.f) atomicAdd(gw2 + m * kGHid + nn, v); });
-        __syncthreads();
-        // dW1 = x^T dA1
-        wave_gemm<kGIn, kGHid, 64>([&](int m, int k) { return M.x[k * kLdX + m]; },
-                                   [&](int k, int nn) { return D.da1[k * kGHid + nn]; },
-                                   [&](int m, int nn, float v) { if (v != 0.f) atomicAdd(gw1 + m * kGHid + nn, v); });
-        // biases: column sums of the (BF16) output and pre-activation gradients
-        {
-            float b3 = 0.f, b2 = 0.f, b1 = 0.f;
-            for (int m = 0; m < 64; m++) {
-                if (lane < kGOutPad) b3 += bf16_to_f32(D.dout[m * kGOutPad + lane]);
-                b2 += bf16_to_f32(D.da2[m * kGHid + lane]);
-                b1 += bf16_to_f32(D.da1[m * kGHid + lane]);
+        __builtin_amdgcn_wave_barrier();
+        // ---- forward: layer 1 (inputs also written transposed for dW1)
+        f32x4 acc[4];
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        _Pragma("unroll") for (int kb = 0; kb < 4; kb++) {
+            const bh8 B = encode_slots(L.raw + ray * kRRow, kb, q);
+            _Pragma("unroll") for (int i = 0; i < 8; i++) L.xt[(32 * kb + 8 * q + i) * kTS + ray] = bits(B[i]);
+            _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gfrag(g.wf, 4 * mt + kb, lane), B, acc[mt], 0, 0, 0);
+        }
+        bh4 h1[4], h2[4], da[4];
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) {
+            h1[mt] = act4(acc[mt], bias4(g.w + kGOffB1 + mt * 16 + 4 * q));
+            acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            _Pragma("unroll") for (int i = 0; i < 4; i++) L.h1t[(mt * 16 + 4 * q + i) * kTS + ray] = bits(h1[mt][i]);
+        }
+        // layer 2
+        _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+            const bh8 B = __builtin_shufflevector(h1[2 * kb], h1[2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+            _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gfrag(g.wf, 16 + 2 * mt + kb, lane), B, acc[mt], 0, 0, 0);
+        }
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) {
+            h2[mt] = act4(acc[mt], bias4(g.w + kGOffB2 + mt * 16 + 4 * q));
+            acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            _Pragma("unroll") for (int i = 0; i < 4; i++) L.h2t[(mt * 16 + 4 * q + i) * kTS + ray] = bits(h2[mt][i]);
+        }
+        // layer 3
+        _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+            const bh8 B = __builtin_shufflevector(h2[2 * kb], h2[2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+            _Pragma("unroll") for (int mt = 0; mt < 3; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gfrag(g.wf, 24 + 2 * mt + kb, lane), B, acc[mt], 0, 0, 0);
+        }
+        _Pragma("unroll") for (int mt = 0; mt < 3; mt++) {
+            const float4 bb = bias4(g.w + kGOffB3 + mt * 16 + 4 * q);
+            const float b4[4] = {bb.x, bb.y, bb.z, bb.w};
+            _Pragma("unroll") for (int i = 0; i < 4; i++) {
+                const int m = mt * 16 + 4 * q + i;
+                if (m < kGOut) L.io[ray * kORow + m] = fadd(acc[mt][i], b4[i]);
             }
-            if (lane < kGOutPad && b3 != 0.f) atomicAdd(gb3 + lane, b3);
-            if (b2 != 0.f) atomicAdd(gb2 + lane, b2);
-            if (b1 != 0.f) atomicAdd(gb1 + lane, b1);
+            acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
+        // ---- output gradient of the ray's record
+        if (q == 0) {
+            float o[kGOut], d[kGOutPad];
+            _Pragma("unroll") for (int k = 0; k < kGOut; k++) o[k] = L.io[ray * kORow + k];
+            if (scale > 0.f) {
+                loss += output_grad(o, wdir, scale, d);
+            } else {
+                _Pragma("unroll") for (int k = 0; k < kGOutPad; k++) d[k] = 0.f;
+            }
+            _Pragma("unroll") for (int k = 0; k < 64; k += 8) {
+                bh8 v;
+                _Pragma("unroll") for (int i = 0; i < 8; i++) v[i] = (__bf16)(k + i < kGOutPad ? d[k + i] : 0.f);
+                *(bh8 *)(L.dor + ray * kTS + k) = v;
+                _Pragma("unroll") for (int i = 0; i < 8; i++)
+                    if (k + i < kGOutPad) L.dot[(k + i) * kTS + ray] = bits(v[i]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // ---- backward: dH2^T = W3 dO^T, masked by relu(layer 2)
+        _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+            const bh8 B = *(const bh8 *)(L.dor + ray * kTS + 32 * kb + 8 * q);
+            _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gfrag(g.wk, 2 * mt + kb, lane), B, acc[mt], 0, 0, 0);
+        }
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++) {
+            _Pragma("unroll") for (int i = 0; i < 4; i++) {
+                da[mt][i] = (__bf16)(bits(h2[mt][i]) ? acc[mt][i] : 0.f);
+                L.da2t[(mt * 16 + 4 * q + i) * kTS + ray] = bits(da[mt][i]);
+            }
+            acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        // dH1^T = W2 dA2^T, masked by relu(layer 1)
+        _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+            const bh8 B = __builtin_shufflevector(da[2 * kb], da[2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+            _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gfrag(g.wk, 8 + 2 * mt + kb, lane), B, acc[mt], 0, 0, 0);
+        }
+        _Pragma("unroll") for (int mt = 0; mt < 4; mt++)
+            _Pragma("unroll") for (int i = 0; i < 4; i++)
+                L.da1t[(mt * 16 + 4 * q + i) * kTS + ray] = bits((__bf16)(bits(h1[mt][i]) ? acc[mt][i] : 0.f));
+        __syncthreads();  // every wave's columns of the transposed copies are written
+        // ---- weight gradients, C[row][col] = sum over the tile's rays: 60 16x16 tiles over the waves
+        for (int T = wv; T < 60; T += kTrainWaves) {
+            const uint16_t *At, *Bt;
+            int mt, nt, base, ld;
+            if (T < 12) {
+                At = L.h2t; Bt = L.dot; mt = T / 3; nt = T % 3; base = kGOffW3; ld = kGOutPad;
+            } else if (T < 28) {
+                At = L.h1t; Bt = L.da2t; mt = (T - 12) / 4; nt = (T - 12) % 4; base = kGOffW2; ld = kGHid;
+            } else {
+                At = L.xt; Bt = L.da1t; mt = (T - 28) / 4; nt = (T - 28) % 4; base = -1; ld = kGHid;
+            }
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            _Pragma("unroll") for (int kb = 0; kb < 2; kb++) {
+                const bh8 a = *(const bh8 *)(At + (mt * 16 + r) * kTS + 32 * kb + 8 * q);
+                const bh8 b = *(const bh8 *)(Bt + (nt * 16 + r) * kTS + 32 * kb + 8 * q);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+            }
+            _Pragma("unroll") for (int i = 0; i < 4; i++) {
+                const int row = mt * 16 + 4 * q + i, col = nt * 16 + r;
+                if (base >= 0) {
+                    L.acc[base + row * ld + col] += c[i];
+                } else {
+                    const int f = guide_slot_feature(row);
+                    if (f >= 0) L.acc[kGOffW1 + f * kGHid + col] += c[i];
+                }
+            }
+        }
+        // bias gradients: row sums of the BF16 output and pre-activation gradients
+        if (tid < 2 * kGHid + kGOutPad) {
+            const uint16_t *row = tid < kGHid ? L.da1t + tid * kTS
+                                  : tid < 2 * kGHid ? L.da2t + (tid - kGHid) * kTS : L.dot + (tid - 2 * kGHid) * kTS;
+            float b = 0.f;
+            for (int k = 0; k < 64; k += 8) {
+                const uint4 x = *(const uint4 *)(row + k);
+                const uint32_t w4[4] = {x.x, x.y, x.z, x.w};
+                _Pragma("unroll") for (int e = 0; e < 4; e++) b += bf((uint16_t)w4[e]) + bf((uint16_t)(w4[e] >> 16));
+            }
+            L.acc[tid < kGHid ? kGOffB1 + tid : tid < 2 * kGHid ? kGOffB2 + tid - kGHid : kGOffB3 + tid - 2 * kGHid] += b;
+        }
     }
+    __syncthreads();
+    float *dst = g.part + (size_t)blockIdx.x * kGParams;
+    for (int p = tid; p < kGParams; p += blockDim.x) stg(dst + p, L.acc[p]);
+    for (int dd = 32; dd >= 1; dd >>= 1) loss += __shfl_xor(loss, dd, 64);
+    if (lane == 0 && loss != 0.f) atomicAdd(g.wsum + 2 * s + 1, loss);
 }
 
-// BF16 copies of the weight matrices, [in][out] and transposed [out][in].
+// BF16 copies of the weight matrices: forward and backward MFMA fragments.
 __device__ __forceinline__ void write_bf16(const GuideNet &g, int p, float v) {
     const uint16_t b = f32_to_bf16(v);
-    uint16_t *wt = g.wb + (kGW1 + kGW2 + kGW3);
-    if (p < kGOffB1) {
-        const int i = p / kGHid, o = p % kGHid;
-        g.wb[p] = b;
-        wt[o * kGIn + i] = b;
-    } else if (p >= kGOffW2 && p < kGOffB2) {
-        const int q = p - kGOffW2, i = q / kGHid, o = q % kGHid;
-        g.wb[kGW1 + q] = b;
-        wt[kGW1 + o * kGHid + i] = b;
-    } else if (p >= kGOffW3 && p < kGOffB3) {
-        const int q = p - kGOffW3, i = q / kGOutPad, o = q % kGOutPad;
-        g.wb[kGW1 + kGW2 + q] = b;
-        wt[kGW1 + kGW2 + o * kGHid + i] = b;
-    }
+    const int fo = guide_frag_offset(p), bo = guide_back_offset(p);
+    if (fo >= 0) g.wf[fo] = b;
+    if (bo >= 0) g.wk[bo] = b;
 }
 
-// Adam step t (1-based) on every parameter; clears the gradient.  The padded
-// output columns of W3/b3 stay zero.
-__global__ void k_guide_adam(GuideNet g, GuideTrain h, int t) {
-    const float c1 = 1.f - powf(h.beta1, (float)t), c2 = 1.f - powf(h.beta2, (float)t);
+// The step's gradient (sum of the kGTrainBlocks rows of g.part) into g.grad;
+// with apply, an Adam step on every parameter.  The padded output columns of
+// W3/b3 stay zero.
+__global__ void k_guide_adam(GuideNet g, GuideTrain h, int apply) {
+    const float c1 = h.c1, c2 = h.c2;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kGParams; p += gridDim.x * blockDim.x) {
-        bool pad = (p >= kGOffW3 && p < kGOffB3 && (p - kGOffW3) % kGOutPad >= kGOut) ||
-                   (p >= kGOffB3 && p - kGOffB3 >= kGOut);
-        const float gr = pad ? 0.f : g.grad[p];
+        float gr = 0.f;
+        _Pragma("unroll") for (int b = 0; b < kGTrainBlocks; b++) gr += ldg(g.part + (size_t)b * kGParams + p);
+        const bool pad = (p >= kGOffW3 && p < kGOffB3 && (p - kGOffW3) % kGOutPad >= kGOut) ||
+                         (p >= kGOffB3 && p - kGOffB3 >= kGOut);
+        gr = pad ? 0.f : gr;
+        g.grad[p] = gr;
+        if (!apply) continue;
         const float m = h.beta1 * g.m[p] + (1.f - h.beta1) * gr;
         const float v = h.beta2 * g.v[p] + (1.f - h.beta2) * gr * gr;
         g.m[p] = m;
         g.v[p] = v;
         const float w = g.w[p] - h.lr * (m / c1) / (sqrtf(v / c2) + h.eps);
         g.w[p] = w;
-        g.grad[p] = 0.f;
         write_bf16(g, p, w);
     }
 }
@@ -469,23 +720,28 @@ __global__ void k_guide_sync(GuideNet g) {
         write_bf16(g, p, g.w[p]);
 }
 
-// Forward pass on given vertices (verification entry): out[n][kGOut].
-__global__ __launch_bounds__(64) void k_guide_forward(GuideNet g, const float *pn, uint32_t n, float *out) {
-    __shared__ WaveMem M;
-    __shared__ NetLds W;
-    if (blockIdx.x * 64u >= n) return;
-    load_weights(g, W);
-    const int lane = (int)__lane_id();
-    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+// Forward pass on given vertices (verification entry): out[n][kGOut], by the
+// same per-wave MLP as k_guide.
+__global__ __launch_bounds__(64 * kGuideWaves) void k_guide_forward(GuideNet g, const float *pn, uint32_t n,
+                                                                    float *out) {
+    __shared__ GuideBlockLds L;
+    if (blockIdx.x * (64u * kGuideWaves) >= n) return;
+    load_frags(g, L.W);
+    const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6);
+    float *io = L.io[wv];
+    for (uint32_t base = (blockIdx.x * kGuideWaves + wv) * 64u; base < n;
+         base += gridDim.x * (64u * kGuideWaves)) {
         const uint32_t i = base + (uint32_t)lane;
         const uint32_t j = i < n ? i : 0u;
-        encode_row(g, v3(pn[6 * j], pn[6 * j + 1], pn[6 * j + 2]), v3(pn[6 * j + 3], pn[6 * j + 4], pn[6 * j + 5]),
-                   M.x + lane * kLdX);
-        __syncthreads();
-        net_forward(g, W, M);
+        float s[6];
+        guide_raw(g, v3(pn[6 * j], pn[6 * j + 1], pn[6 * j + 2]), v3(pn[6 * j + 3], pn[6 * j + 4], pn[6 * j + 5]),
+                  s);
+        _Pragma("unroll") for (int d = 0; d < 6; d++) io[kRawOff + lane * kRRow + d] = s[d];
+        __builtin_amdgcn_wave_barrier();
+        mlp_wave(L.W, io);
         if (i < n)
-            for (int k = 0; k < kGOut; k++) out[(size_t)i * kGOut + k] = M.o[lane * kGOutPad + k];
-        __syncthreads();
+            for (int k = 0; k < kGOut; k++) out[(size_t)i * kGOut + k] = io[lane * kORow + k];
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -494,7 +750,7 @@ __global__ __launch_bounds__(64) void k_guide_forward(GuideNet g, const float *p
 // ------------------------------------------------------------- launchers --
 using namespace pcr;
 extern "C" hipError_t pcr_launch_guide(const GuideArgs *a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide, dim3(grid), dim3(64), 0, s, *a);
+    hipLaunchKernelGGL(k_guide, dim3(grid), dim3(64 * kGuideWaves), 0, s, *a);
     return hipGetLastError();
 }
 extern "C" hipError_t pcr_launch_guide_prep(const GuideNet *g, GuideRecord *rec, uint32_t rec_cap,
@@ -502,13 +758,13 @@ extern "C" hipError_t pcr_launch_guide_prep(const GuideNet *g, GuideRecord *rec,
     hipLaunchKernelGGL(k_guide_prep, dim3(256), dim3(256), 0, s, *g, rec, rec_cap, radiance, S);
     return hipGetLastError();
 }
-extern "C" hipError_t pcr_launch_guide_grad(const GuideNet *g, const GuideRecord *rec, uint32_t rec_cap, int step,
-                                            int S, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_grad, dim3(grid), dim3(64), 0, s, *g, rec, rec_cap, step, S);
+extern "C" hipError_t pcr_launch_guide_train(const GuideNet *g, const GuideRecord *rec, uint32_t rec_cap, int step,
+                                             int S, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_train, dim3(kGTrainBlocks), dim3(64 * kTrainWaves), 0, s, *g, rec, rec_cap, step, S);
     return hipGetLastError();
 }
-extern "C" hipError_t pcr_launch_guide_adam(const GuideNet *g, const GuideTrain *h, int t, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, *g, *h, t);
+extern "C" hipError_t pcr_launch_guide_adam(const GuideNet *g, const GuideTrain *h, int apply, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, *g, *h, apply);
     return hipGetLastError();
 }
 extern "C" hipError_t pcr_launch_guide_sync(const GuideNet *g, hipStream_t s) {
@@ -517,6 +773,6 @@ extern "C" hipError_t pcr_launch_guide_sync(const GuideNet *g, hipStream_t s) {
 }
 extern "C" hipError_t pcr_launch_guide_forward(const GuideNet *g, const float *pn, uint32_t n, float *out, int grid,
                                                hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(64), 0, s, *g, pn, n, out);
+    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(64 * kGuideWaves), 0, s, *g, pn, n, out);
     return hipGetLastError();
 }

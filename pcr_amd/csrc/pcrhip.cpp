@@ -215,7 +215,7 @@ int validate_job(const pcrhip_scene *s, const pcrhip_job_desc *d) {
     if ((uint64_t)d->width * d->height * (uint64_t)d->spp >= (1ull << 62))
         return fail(PCRHIP_EINVAL, "too many samples");
     if (s->S.n_lights < 1) return fail(PCRHIP_EINVAL, "scene has no light");
-    if (d->guide_steps < 0 || d->guide_steps > kGMaxSteps) return fail(PCRHIP_EINVAL, "guide_steps must be in 0..16");
+    if (d->guide_steps < 0 || d->guide_steps > kGMaxSteps) return fail(PCRHIP_EINVAL, "guide_steps must be in 0..128");
     if (!(d->guide_lr >= 0.f) || d->guide_records < 0) return fail(PCRHIP_EINVAL, "bad guide_lr or guide_records");
     return PCRHIP_OK;
 }
@@ -317,30 +317,46 @@ std::vector<float> guide_init(uint64_t seed) {
     return w;
 }
 
-// One allocation holds the job's network state; caller holds c->m.
-hipError_t guide_alloc(pcrhip_ctx *c, pcrhip_job *j) {
-    const size_t P = kGParams, nb = 2 * (size_t)(kGW1 + kGW2 + kGW3);
-    const size_t bytes = 4 * P * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 + nb * sizeof(uint16_t);
-    char *p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, bytes);
-    if (e != hipSuccess) return e;
-    j->net_mem = p;
-    GuideNet &g = j->net;
-    g.w = (float *)p;
+// One device allocation holds a network's state: FP32 weights, gradient and
+// Adam moments, per-step sums, counters, and the BF16 copies.
+// Every region starts 16-byte aligned (the fragment loads are 16 bytes).
+size_t guide_bytes() {
+    return (4 + (size_t)kGTrainBlocks) * kGParams * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 +
+           (size_t)(kGFragElems + kGBackElems) * sizeof(uint16_t);
+}
+void guide_layout(GuideNet &g, void *mem) {
+    const size_t P = kGParams;
+    g.w = (float *)mem;
     g.grad = g.w + P;
     g.m = g.grad + P;
     g.v = g.m + P;
     g.wsum = g.v + P;
     g.n_rec = (uint32_t *)(g.wsum + 2 * kGMaxSteps);
     g.step = (int32_t *)(g.n_rec + 1);
-    g.wb = (uint16_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+    g.wf = (uint16_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+    g.wk = g.wf + kGFragElems;
+    g.part = (float *)(g.wk + kGBackElems);
+}
+static_assert((4 * kGParams * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64) % 16 == 0 &&
+                  (kGFragElems * sizeof(uint16_t)) % 16 == 0 && (kGBackElems * sizeof(uint16_t)) % 16 == 0,
+              "guide regions stay 16-byte aligned");
+
+// One allocation holds the job's network state; caller holds c->m.
+hipError_t guide_alloc(pcrhip_ctx *c, pcrhip_job *j) {
+    const size_t bytes = guide_bytes();
+    char *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess) return e;
+    j->net_mem = p;
+    GuideNet &g = j->net;
+    guide_layout(g, p);
     for (int k = 0; k < 3; k++) {
         g.lo[k] = j->scene->box_lo[k];
         g.inv[k] = 1.f / (j->scene->box_hi[k] - j->scene->box_lo[k]);
     }
     std::vector<float> w0 = guide_init(j->d.seed);
     if ((e = hipMemsetAsync(p, 0, bytes, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(g.w, w0.data(), P * sizeof(float), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(g.w, w0.data(), kGParams * sizeof(float), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return e;
     if ((e = pcr_launch_guide_sync(&g, c->stream)) != hipSuccess) return e;
     j->guided = true;
@@ -491,18 +507,21 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
         HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, sh.grid, sh.block, c->stream));
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
-        GuideArgs ga{g, a.out, a.count_out, 4u * segs[0].count, c->grec, rec_cap, keep};
-        HIPCHK(pcr_launch_guide(&ga, c->n_cu * 2, c->stream));
+        GuideArgs ga{g, a.out, a.count_out, 4u * segs[0].count, c->grec, rec_cap, keep, j->adam_t > 0};
+        HIPCHK(pcr_launch_guide(&ga, c->n_cu, c->stream));
     }
     ResolveArgs ra{c->jobs_dev, c->rsegs_dev, c->pass_dev};
     if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb], c->stream));
     HIPCHK(pcr_launch_resolve(&ra, sh.rgrid, c->stream));
     if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb + 1], c->stream));
     HIPCHK(pcr_launch_guide_prep(&g, c->grec, rec_cap, c->pass_rad + segs[0].begin, S, c->stream));
-    const GuideTrain h{j->d.guide_lr > 0.f ? j->d.guide_lr : 5e-3f, 0.9f, 0.99f, 1e-8f};
+    GuideTrain h{j->d.guide_lr > 0.f ? j->d.guide_lr : 5e-3f, 0.9f, 0.99f, 1e-8f, 0.f, 0.f};
     for (int q = 0; q < S; q++) {
-        HIPCHK(pcr_launch_guide_grad(&g, c->grec, rec_cap, q, S, c->n_cu, c->stream));
-        HIPCHK(pcr_launch_guide_adam(&g, &h, ++j->adam_t, c->stream));
+        const int t = ++j->adam_t;
+        h.c1 = (float)(1.0 - std::pow((double)h.beta1, t));
+        h.c2 = (float)(1.0 - std::pow((double)h.beta2, t));
+        HIPCHK(pcr_launch_guide_train(&g, c->grec, rec_cap, q, S, c->stream));
+        HIPCHK(pcr_launch_guide_adam(&g, &h, 1, c->stream));
     }
     return PCRHIP_OK;
 }
@@ -606,7 +625,8 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         c->pass_rad_cap = cap;
     }
     for (size_t i = 0; i < nj; i++) c->jobs_host[jobs[i]->slot].radiance = c->pass_rad + segs[i].begin;
-    const uint32_t rec_cap = guided ? (uint32_t)(jobs[0]->d.guide_records > 0 ? jobs[0]->d.guide_records : 65536) : 0;
+    const uint32_t rec_cap =
+        guided ? (uint32_t)(jobs[0]->d.guide_records > 0 ? jobs[0]->d.guide_records : kGDefaultRecords) : 0;
     if (rec_cap > c->grec_cap) {
         (void)hipFree(c->grec);
         c->grec = nullptr;
@@ -630,7 +650,7 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     // launches (same kernels, same grid).
     const bool prof = c->profile.load();
     std::vector<hipEvent_t> ev;
-    const int S = guided ? (jobs[0]->d.guide_steps > 0 ? jobs[0]->d.guide_steps : 16) : 0;
+    const int S = guided ? (jobs[0]->d.guide_steps > 0 ? jobs[0]->d.guide_steps : kGDefaultSteps) : 0;
     if (guided) {
         if (prof && (rc = make_events(ev, 2 * nb + 2))) return rc;
         if ((rc = enqueue_guided_pass(c, sh, ev, (uint32_t)n_gen, segs, jobs[0], rec_cap, S))) return rc;
@@ -1420,23 +1440,15 @@ struct TempNet {
     int init(pcrhip_ctx *c, const float *params, const float *box) {
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        const size_t P = kGParams, nb = 2 * (size_t)(kGW1 + kGW2 + kGW3);
-        const size_t bytes = 4 * P * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 + nb * sizeof(uint16_t);
+        const size_t bytes = guide_bytes();
         HIPCHK(hipMalloc(&mem, bytes));
         HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
-        g.w = (float *)mem;
-        g.grad = g.w + P;
-        g.m = g.grad + P;
-        g.v = g.m + P;
-        g.wsum = g.v + P;
-        g.n_rec = (uint32_t *)(g.wsum + 2 * kGMaxSteps);
-        g.step = (int32_t *)(g.n_rec + 1);
-        g.wb = (uint16_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+        guide_layout(g, mem);
         for (int k = 0; k < 3; k++) {
             g.lo[k] = box[k];
             g.inv[k] = 1.f / (box[3 + k] - box[k]);
         }
-        HIPCHK(hipMemcpyAsync(g.w, params, P * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(g.w, params, kGParams * sizeof(float), hipMemcpyHostToDevice, st));
         HIPCHK(pcr_launch_guide_sync(&g, st));
         return PCRHIP_OK;
     }
@@ -1460,12 +1472,15 @@ int pcrhip_guide_forward(pcrhip_ctx *c, const float *params, const float *box, c
     if ((e = hipMalloc(&dv, (size_t)n * 6 * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&dout, (size_t)n * kGOut * sizeof(float))) != hipSuccess ||
         (e = hipMemcpyAsync(dv, verts, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
-        (e = pcr_launch_guide_forward(&t.g, dv, (uint32_t)n, dout, c->n_cu * 2, t.st)) != hipSuccess ||
+        (e = pcr_launch_guide_forward(&t.g, dv, (uint32_t)n, dout, c->n_cu, t.st)) != hipSuccess ||
         (e = hipMemcpyAsync(out, dout, (size_t)n * kGOut * sizeof(float), hipMemcpyDeviceToHost, t.st)) != hipSuccess ||
         (e = hipStreamSynchronize(t.st)) != hipSuccess)
         return done(fail(PCRHIP_EDEVICE, std::string("guide forward: ") + hipGetErrorString(e)));
     return done(PCRHIP_OK);
 }
+
+// gradient only: the Adam kernel sums the per-block rows and leaves the weights
+static const GuideTrain kNoTrain{0.f, 0.9f, 0.99f, 1e-8f, 1.f, 1.f};
 
 int pcrhip_guide_gradient(pcrhip_ctx *c, const float *params, const float *box, const float *recs, int n,
                           float *grad, float *loss) {
@@ -1502,7 +1517,8 @@ int pcrhip_guide_gradient(pcrhip_ctx *c, const float *params, const float *box, 
         (e = hipMemcpyAsync(dr, hr.data(), (size_t)n * sizeof(GuideRecord), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
         (e = hipMemcpyAsync(t.g.wsum, wsum, sizeof(wsum), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
         (e = hipMemcpyAsync(t.g.n_rec, &nr, sizeof(nr), hipMemcpyHostToDevice, t.st)) != hipSuccess ||
-        (e = pcr_launch_guide_grad(&t.g, dr, nr, 0, 1, c->n_cu, t.st)) != hipSuccess ||
+        (e = pcr_launch_guide_train(&t.g, dr, nr, 0, 1, t.st)) != hipSuccess ||
+        (e = pcr_launch_guide_adam(&t.g, &kNoTrain, 0, t.st)) != hipSuccess ||
         (e = hipMemcpyAsync(grad, t.g.grad, kGParams * sizeof(float), hipMemcpyDeviceToHost, t.st)) != hipSuccess ||
         (e = hipMemcpyAsync(outw, t.g.wsum, sizeof(outw), hipMemcpyDeviceToHost, t.st)) != hipSuccess ||
         (e = hipStreamSynchronize(t.st)) != hipSuccess)
