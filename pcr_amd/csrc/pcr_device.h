@@ -100,6 +100,7 @@ __device__ __forceinline__ int2 ldg(const int2 *p) {
 }
 __device__ __forceinline__ float ldg(const float *p) { return *(const PCR_GLOBAL float *)(const void *)p; }
 __device__ __forceinline__ int32_t ldg(const int32_t *p) { return *(const PCR_GLOBAL int32_t *)(const void *)p; }
+__device__ __forceinline__ uint32_t ldg(const uint32_t *p) { return *(const PCR_GLOBAL uint32_t *)(const void *)p; }
 __device__ __forceinline__ double ldg(const double *p) { return *(const PCR_GLOBAL double *)(const void *)p; }
 __device__ __forceinline__ void stg(float4 *p, float4 v) {
     *(PCR_GLOBAL pcr_f4_t *)(void *)p = pcr_f4_t{v.x, v.y, v.z, v.w};

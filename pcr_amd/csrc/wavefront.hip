@@ -333,11 +333,13 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
             HitRec sh = scene_closest<STACK>(S, so, wi, tb, st);
             bool occl = false;
             if (sh.kind >= 0) {
-                V3 sp, sn;
-                int sm;
-                hit_geometry(S, sh, so, wi, sp, sn, sm);
-                V3 dd = sub(sp, so);
-                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(ldg(S.mat + sm).flags & MF_EMISSIVE);
+                // the occlusion rule needs the hit point (as hit_geometry
+                // computes it) and the material's emissive flag, not the normal
+                const int sm = sh.kind == 0   ? ldg(S.sph_mat + sh.prim)
+                               : sh.kind == 1 ? __float_as_int(ldg(S.pl + 4 * sh.prim + 2).w)
+                                              : __float_as_int(ldg(S.tri + 3 * sh.prim).w);
+                const V3 dd = sub(add(so, mul(wi, sh.t)), so);
+                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(ldg(&S.mat[sm].flags) & MF_EMISSIVE);
             }
             if (!occl) {
                 float cosL = smax(0.f, dot(sN, mul(Li, -1.f)));
