@@ -102,6 +102,26 @@ __device__ __forceinline__ float ldg(const float *p) { return *(const PCR_GLOBAL
 __device__ __forceinline__ int32_t ldg(const int32_t *p) { return *(const PCR_GLOBAL int32_t *)(const void *)p; }
 __device__ __forceinline__ uint32_t ldg(const uint32_t *p) { return *(const PCR_GLOBAL uint32_t *)(const void *)p; }
 __device__ __forceinline__ double ldg(const double *p) { return *(const PCR_GLOBAL double *)(const void *)p; }
+// Geometry reads of the traversal: from global memory, or from a copy of the
+// scene in LDS (the pointers then point into LDS; the cast to the local
+// address space turns them into ds_read offsets).
+#define PCR_LDS __attribute__((address_space(3)))
+struct GeoGlobal {
+    static __device__ __forceinline__ float4 f4(const float4 *p) { return ldg(p); }
+    static __device__ __forceinline__ int2 i2(const int2 *p) { return ldg(p); }
+    static __device__ __forceinline__ int32_t i1(const int32_t *p) { return ldg(p); }
+};
+struct GeoLds {
+    static __device__ __forceinline__ float4 f4(const float4 *p) {
+        const pcr_f4_t v = *(const PCR_LDS pcr_f4_t *)(const void *)p;
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+    static __device__ __forceinline__ int2 i2(const int2 *p) {
+        const pcr_i2_t v = *(const PCR_LDS pcr_i2_t *)(const void *)p;
+        return make_int2(v.x, v.y);
+    }
+    static __device__ __forceinline__ int32_t i1(const int32_t *p) { return *(const PCR_LDS int32_t *)(const void *)p; }
+};
 __device__ __forceinline__ void stg(float4 *p, float4 v) {
     *(PCR_GLOBAL pcr_f4_t *)(void *)p = pcr_f4_t{v.x, v.y, v.z, v.w};
 }
@@ -272,9 +292,10 @@ __device__ __forceinline__ float cauchy_ior(float ior, float B, float l) {
 
 // ------------------------------------------------------- intersections -----
 // Moller-Trumbore; on success updates closest and the (triangle, u, v) record.
+template <class G = GeoGlobal>
 __device__ __forceinline__ bool tri_test(const float4 *__restrict__ tri, int ti, V3 o, V3 d, float &closest,
                                          int &best, float &bu, float &bv) {
-    float4 a = ldg(tri + 3 * ti), b = ldg(tri + 3 * ti + 1), c = ldg(tri + 3 * ti + 2);
+    float4 a = G::f4(tri + 3 * ti), b = G::f4(tri + 3 * ti + 1), c = G::f4(tri + 3 * ti + 2);
     V3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
@@ -349,7 +370,7 @@ struct LdsStack {
 // the near child is taken directly and only the far one is pushed, which is the
 // same visiting order.  Pushes that the reference's 64-entry stack would drop
 // are dropped here too (never reached for trees shallower than 63).
-template <int STACK>
+template <int STACK, class G = GeoGlobal>
 __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, float tmax, const LdsStack &st) {
     HitRec h;
     h.kind = -1;
@@ -357,7 +378,7 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
     h.u = h.v = 0.f;
     float closest = tmax;
     for (int i = 0; i < S.n_sph; i++) {
-        float4 s = ldg(S.sph + i);
+        float4 s = G::f4(S.sph + i);
         V3 cp = sub(xyz(s), o);
         float rl = dot(d, cp);
         float tSq = fsub(dot(cp, cp), fmul(rl, rl));
@@ -374,8 +395,8 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
         h.prim = i;
     }
     for (int i = 0; i < S.n_pl; i++) {
-        float4 po = ldg(S.pl + 4 * i), pu = ldg(S.pl + 4 * i + 1), pv = ldg(S.pl + 4 * i + 2),
-               pn = ldg(S.pl + 4 * i + 3);
+        float4 po = G::f4(S.pl + 4 * i), pu = G::f4(S.pl + 4 * i + 1), pv = G::f4(S.pl + 4 * i + 2),
+               pn = G::f4(S.pl + 4 * i + 3);
         V3 N = xyz(pn);
         float denom = dot(d, N);
         if (fabsf(denom) <= 1e-6f) continue;
@@ -402,11 +423,11 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
             int cur = S.root_ref;
             for (;;) {
                 if (cur < 0) {
-                    int2 lf = ldg(S.leaves + ~cur);
-                    for (int k = 0; k < lf.y; k++) tri_test(S.tri, lf.x + k, o, d, closest, best, bu, bv);
+                    int2 lf = G::i2(S.leaves + ~cur);
+                    for (int k = 0; k < lf.y; k++) tri_test<G>(S.tri, lf.x + k, o, d, closest, best, bu, bv);
                 } else {
                     const float4 *nd = S.bvh + 4 * cur;
-                    float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), e = ldg(nd + 3);
+                    float4 a = G::f4(nd), b = G::f4(nd + 1), c = G::f4(nd + 2), e = G::f4(nd + 3);
                     float tL, tR;
                     bool hL = box_test(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, closest, tL);
                     bool hR = box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, closest, tR);
@@ -454,16 +475,17 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
 
 // Hit point and unflipped normal for a hit record (Renderer.cpp:905-906,
 // Plane.h:259, Triangle.h:343-357): hit = o + d * t for every primitive kind.
+template <class G = GeoGlobal>
 __device__ __forceinline__ void hit_geometry(const DScene &S, const HitRec &h, V3 o, V3 d, V3 &P, V3 &N, int &mat) {
     P = add(o, mul(d, h.t));
     if (h.kind == 0) {
-        N = normalize(sub(P, xyz(ldg(S.sph + h.prim))));
-        mat = ldg(S.sph_mat + h.prim);
+        N = normalize(sub(P, xyz(G::f4(S.sph + h.prim))));
+        mat = G::i1(S.sph_mat + h.prim);
     } else if (h.kind == 1) {
-        N = xyz(ldg(S.pl + 4 * h.prim + 3));
-        mat = __float_as_int(ldg(S.pl + 4 * h.prim + 2).w);
+        N = xyz(G::f4(S.pl + 4 * h.prim + 3));
+        mat = __float_as_int(G::f4(S.pl + 4 * h.prim + 2).w);
     } else {
-        float4 a = ldg(S.tri + 3 * h.prim), b = ldg(S.tri + 3 * h.prim + 1);
+        float4 a = G::f4(S.tri + 3 * h.prim), b = G::f4(S.tri + 3 * h.prim + 1);
         mat = __float_as_int(a.w);
         const float4 *tn = S.tri_n + 4 * h.prim;
         if (__float_as_int(b.w)) {

@@ -60,6 +60,26 @@ struct PassInfo {
     int32_t n_rsegs;   // ResolveSeg entries
 };
 
+// A small scene's traversal data copied into LDS by every block of a bounce
+// launch (float4 units): spheres, sphere materials (4 per float4), plane
+// records, triangles, inner nodes, leaves (2 per float4).  Used when all jobs
+// of a pass share one scene that fits kGeoF4.
+constexpr int kGeoF4 = 512;
+struct GeoLayout {
+    int sph, sphm, pl, tri, bvh, leaf, total;
+};
+__host__ __device__ inline GeoLayout geo_layout(int n_sph, int n_pl, int n_tri, int n_inner, int n_leaves) {
+    GeoLayout g;
+    g.sph = 0;
+    g.sphm = n_sph;
+    g.pl = g.sphm + (n_sph + 3) / 4;
+    g.tri = g.pl + 4 * n_pl;
+    g.bvh = g.tri + 3 * n_tri;
+    g.leaf = g.bvh + 4 * n_inner;
+    g.total = g.leaf + (n_leaves + 1) / 2;
+    return g;
+}
+
 struct BounceArgs {
     const DJob *jobs;
     const GenSeg *segs;
@@ -92,7 +112,10 @@ struct IntersectArgs {
 }  // namespace pcr
 
 extern "C" {
-hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block, hipStream_t s);
+// lds_scene: every job of the pass uses one scene whose geo_layout fits
+// kGeoF4 (stacks of at most 16 entries); the blocks read it from LDS.
+hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block, int lds_scene,
+                             hipStream_t s);
 // Threads per block the bounce launcher actually uses for a stack depth.
 int pcr_bounce_block(int stack, int block);
 hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, int grid, hipStream_t s);

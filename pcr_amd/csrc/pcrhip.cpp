@@ -87,6 +87,7 @@ struct pcrhip_scene {
     int stack = 8;
     int depth = 0;
     int n_inner = 0, n_leaves = 0;
+    bool lds_scene = false;  // traversal data fit kGeoF4 and the stack is at most 16 entries
     uint64_t bytes = 0;
     std::atomic<int> jobs{0};
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // bounds of all geometry
@@ -145,6 +146,7 @@ struct pcrhip_ctx {
     // captured pass graphs, keyed by launch shape; dropped when the queues move
     struct PassGraph {
         int nb, stack, block, grid;
+        bool lds_scene;
         hipGraphExec_t exec;
     };
     std::vector<PassGraph> graphs;
@@ -430,6 +432,7 @@ int pass_class(const pcrhip_job *j) {
 // Launch shape of one pass.
 struct PassShape {
     int nb, stack, block, grid, rgrid;
+    bool lds_scene;  // the pass's one scene is read from LDS (geo_layout fits kGeoF4)
 };
 
 BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
@@ -473,7 +476,7 @@ int enqueue_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hipEvent_
             grid = (int)std::min<uint64_t>((live + eb - 1) / eb, 1u << 30);
         }
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
-        HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, grid, sh.block, c->stream));
+        HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, grid, sh.block, sh.lds_scene, c->stream));
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
     }
     ResolveArgs ra{c->jobs_dev, c->rsegs_dev, c->pass_dev};
@@ -505,7 +508,7 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
     for (int b = 0; b < sh.nb; b++) {
         BounceArgs a = bounce_args(c, sh, b);
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
-        HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, sh.grid, sh.block, c->stream));
+        HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, sh.grid, sh.block, sh.lds_scene, c->stream));
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
         GuideArgs ga{g, a.out, a.count_out, 4u * segs[0].count, c->grec, rec_cap, keep, j->adam_t > 0};
         HIPCHK(pcr_launch_guide(&ga, c->n_cu, c->stream));
@@ -529,11 +532,12 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
 // The captured graph of a pass shape, built on first use.
 int pass_graph(pcrhip_ctx *c, const PassShape &sh, pcrhip_ctx::PassGraph **out) {
     for (auto &g : c->graphs)
-        if (g.nb == sh.nb && g.stack == sh.stack && g.block == sh.block && g.grid == sh.grid) {
+        if (g.nb == sh.nb && g.stack == sh.stack && g.block == sh.block && g.grid == sh.grid &&
+            g.lds_scene == sh.lds_scene) {
             *out = &g;
             return PCRHIP_OK;
         }
-    pcrhip_ctx::PassGraph g{sh.nb, sh.stack, sh.block, sh.grid, nullptr};
+    pcrhip_ctx::PassGraph g{sh.nb, sh.stack, sh.block, sh.grid, sh.lds_scene, nullptr};
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     int rc = enqueue_pass(c, sh, {}, false, 0, {});
@@ -642,8 +646,10 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     HIPCHK(hipMemcpyAsync(c->rsegs_dev, rsegs.data(), rsegs.size() * sizeof(ResolveSeg), hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipMemcpyAsync(c->pass_dev, c->pass_host, sizeof(PassInfo), hipMemcpyHostToDevice, c->stream));
+    bool one_scene = true;
+    for (auto *j : jobs) one_scene = one_scene && j->scene == jobs[0]->scene;
     const PassShape sh{std::max(max_depth, 1), stack, cfg.block, c->n_cu * std::max(1, cfg.blocks_per_cu),
-                       c->n_cu * 4};
+                       c->n_cu * 4, one_scene && jobs[0]->scene->lds_scene};
     const int nb = sh.nb;
     // Per-kernel event timing needs the launches on the stream itself, so a
     // profiled pass of the graph schedule is issued as the same persistent
@@ -989,6 +995,7 @@ int build_scene(pcrhip_scene *s, const pcrhip_scene_desc *d) {
     S.n_mat = d->n_materials;
     S.n_lights = d->n_lights;
     S.total_area = total;
+    s->lds_scene = s->stack <= 16 && geo_layout(S.n_sph, S.n_pl, S.n_tri, S.n_inner, S.n_leaves).total <= kGeoF4;
     S.cam = make_float3(d->cam_pos[0], d->cam_pos[1], d->cam_pos[2]);
     S.fov = d->cam_fov;
     // bounds of all geometry (input normalisation of the guiding network)

@@ -20,6 +20,8 @@
 // tracePathSpectralSingle) evaluated with the reference CPU's material and
 // intersection rules (code/Renderer.cpp:468-949, code/Includes/Material.h).
 // The per-path arithmetic matches the oracle's orc_render_paths exactly.
+#include <type_traits>
+
 #include "pcr_device.h"
 #include "path_state.h"
 
@@ -154,16 +156,16 @@ struct ForkSink {
 
 // One bounce of one path.  Updates `ps` in place and returns true if it
 // continues as a single ray; terminated paths (and forks, whose sub-paths go
-// to the sink) return false.
-template <int STACK>
-__device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const LdsStack &st, BounceCounters &cnt,
-                                            const ForkSink &fork) {
+// to the sink) return false.  S is the job's scene; its traversal data are
+// read through GA (global memory, or the launch's LDS copy).
+template <int STACK, class GA>
+__device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, PathState &ps, const LdsStack &st,
+                                            BounceCounters &cnt, const ForkSink &fork) {
     if (ps.meta & ST_DEAD) {  // ended by the guiding kernel (direction below the surface)
         write_radiance(J, ps);
         return false;
     }
     const int slot = st_slot(ps.meta);
-    const DScene &S = J.S;
     const bool spectral = (J.flags & JF_SPECTRAL) != 0;
     const int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
     const int bounce = st_bounce(ps.meta);
@@ -181,14 +183,14 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
     }
 
     cnt.ext++;
-    HitRec h = scene_closest<STACK>(S, ps.o, ps.d, 3.40282347e38f, st);
+    HitRec h = scene_closest<STACK, GA>(S, ps.o, ps.d, 3.40282347e38f, st);
     if (h.kind < 0) {
         write_radiance(J, ps);
         return false;
     }
     V3 hit, N;
     int mi;
-    hit_geometry(S, h, ps.o, ps.d, hit, N, mi);
+    hit_geometry<GA>(S, h, ps.o, ps.d, hit, N, mi);
     bool entering = dot(ps.d, N) < 0.f;
     if (!entering) N = mul(N, -1.f);
     const DMat M = ldg(S.mat + mi);
@@ -330,14 +332,14 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, PathState &ps, const 
             // occlusion answer of the reference's unbounded search -- unchanged.
             float tb = fadd(fmul(sqrt_rn(d2), 1.001f), 1.f);
             if (!(tb < 3.40282347e38f)) tb = 3.40282347e38f;
-            HitRec sh = scene_closest<STACK>(S, so, wi, tb, st);
+            HitRec sh = scene_closest<STACK, GA>(S, so, wi, tb, st);
             bool occl = false;
             if (sh.kind >= 0) {
                 // the occlusion rule needs the hit point (as hit_geometry
                 // computes it) and the material's emissive flag, not the normal
-                const int sm = sh.kind == 0   ? ldg(S.sph_mat + sh.prim)
-                               : sh.kind == 1 ? __float_as_int(ldg(S.pl + 4 * sh.prim + 2).w)
-                                              : __float_as_int(ldg(S.tri + 3 * sh.prim).w);
+                const int sm = sh.kind == 0   ? GA::i1(S.sph_mat + sh.prim)
+                               : sh.kind == 1 ? __float_as_int(GA::f4(S.pl + 4 * sh.prim + 2).w)
+                                              : __float_as_int(GA::f4(S.tri + 3 * sh.prim).w);
                 const V3 dd = sub(add(so, mul(wi, sh.t)), so);
                 occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(ldg(&S.mat[sm].flags) & MF_EMISSIVE);
             }
@@ -455,7 +457,10 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // waves (128 VGPRs, ~200 bytes of spills) slower.  Up to 24 stack entries,
 // 3 waves per SIMD also fit LDS (12 waves x 64 lanes x 24 x 8 bytes = 147 KB
 // of 160 KB); deeper stacks are limited by LDS and keep the compiler's choice.
-template <bool FIRST, int STACK, int BLOCK>
+// LDS_SCENE: every job of the pass shares one scene small enough for kGeoF4;
+// each block copies its traversal data (geo_layout) into LDS first and the
+// traversal reads them there.
+template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 24 ? 3 : 1, STACK <= 24 ? 3 : 10)))
 void k_bounce(BounceArgs a) {
     __shared__ int32_t s_idx[STACK * BLOCK];
@@ -463,9 +468,22 @@ void k_bounce(BounceArgs a) {
     __shared__ uint32_t s_base;
     __shared__ uint32_t s_pref[kMaxSegs + 1];
     __shared__ GenSeg s_segs[kMaxSegs];
+    __shared__ float4 s_geo[LDS_SCENE ? kGeoF4 : 1];
+    using GA = typename std::conditional<LDS_SCENE, GeoLds, GeoGlobal>::type;
     LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
     const int n_segs = a.pass->n_segs;
     const uint32_t lane = lane_id();
+    GeoLayout gl{};
+    if (LDS_SCENE) {
+        const DScene &S0 = a.jobs[a.segs[0].slot].S;
+        gl = geo_layout(S0.n_sph, S0.n_pl, S0.n_tri, S0.n_inner, S0.n_leaves);
+        for (int i = threadIdx.x; i < S0.n_sph; i += BLOCK) s_geo[gl.sph + i] = ldg(S0.sph + i);
+        for (int i = threadIdx.x; i < S0.n_sph; i += BLOCK) ((int32_t *)(s_geo + gl.sphm))[i] = ldg(S0.sph_mat + i);
+        for (int i = threadIdx.x; i < 4 * S0.n_pl; i += BLOCK) s_geo[gl.pl + i] = ldg(S0.pl + i);
+        for (int i = threadIdx.x; i < 3 * S0.n_tri; i += BLOCK) s_geo[gl.tri + i] = ldg(S0.tri + i);
+        for (int i = threadIdx.x; i < 4 * S0.n_inner; i += BLOCK) s_geo[gl.bvh + i] = ldg(S0.bvh + i);
+        for (int i = threadIdx.x; i < S0.n_leaves; i += BLOCK) ((int2 *)(s_geo + gl.leaf))[i] = ldg(S0.leaves + i);
+    }
     // The segment table and the exclusive prefix of the segments' input
     // counts, staged in LDS (one wave scans <= 64 segments): every ray's queue
     // address depends on them, so they must not cost a memory round trip.
@@ -535,7 +553,18 @@ void k_bounce(BounceArgs a) {
                 if (FIRST) gen_path(J, s0, gpath, ps);
                 const ForkSink fork{&a.out, a.count_out + k0, 4u * g_begin, 4u * g_count, a.overflow};
                 BounceCounters cnt{0u, 0u, 0u};
-                alive = bounce_path<STACK>(J, ps, st, cnt, fork);
+                if (LDS_SCENE) {
+                    DScene Sl = J.S;
+                    Sl.sph = s_geo + gl.sph;
+                    Sl.sph_mat = (const int32_t *)(s_geo + gl.sphm);
+                    Sl.pl = s_geo + gl.pl;
+                    Sl.tri = s_geo + gl.tri;
+                    Sl.bvh = s_geo + gl.bvh;
+                    Sl.leaves = (const int2 *)(s_geo + gl.leaf);
+                    alive = bounce_path<STACK, GA>(J, Sl, ps, st, cnt, fork);
+                } else {
+                    alive = bounce_path<STACK, GA>(J, J.S, ps, st, cnt, fork);
+                }
                 lc.paths += FIRST ? 1u : 0u;
                 lc.ext += cnt.ext;
                 lc.shadow += cnt.shadow;
@@ -670,16 +699,21 @@ __global__ void k_reset(uint32_t *p, int n) {
 }
 
 // ------------------------------------------------------------- launchers --
-template <bool FIRST, int STACK, int BLOCK>
+template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
 static hipError_t launch_bounce_t(const BounceArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_bounce<FIRST, STACK, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_bounce<FIRST, STACK, BLOCK, LDS_SCENE>), dim3(grid), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
-template <int STACK>
+template <int STACK, bool LDS_SCENE>
 static hipError_t launch_bounce_s(const BounceArgs &a, bool first, int grid, int block, hipStream_t s) {
-    if (block == 64) return first ? launch_bounce_t<true, STACK, 64>(a, grid, s) : launch_bounce_t<false, STACK, 64>(a, grid, s);
-    if (block == 128) return first ? launch_bounce_t<true, STACK, 128>(a, grid, s) : launch_bounce_t<false, STACK, 128>(a, grid, s);
-    return first ? launch_bounce_t<true, STACK, 256>(a, grid, s) : launch_bounce_t<false, STACK, 256>(a, grid, s);
+    if (block == 64)
+        return first ? launch_bounce_t<true, STACK, 64, LDS_SCENE>(a, grid, s)
+                     : launch_bounce_t<false, STACK, 64, LDS_SCENE>(a, grid, s);
+    if (block == 128)
+        return first ? launch_bounce_t<true, STACK, 128, LDS_SCENE>(a, grid, s)
+                     : launch_bounce_t<false, STACK, 128, LDS_SCENE>(a, grid, s);
+    return first ? launch_bounce_t<true, STACK, 256, LDS_SCENE>(a, grid, s)
+                 : launch_bounce_t<false, STACK, 256, LDS_SCENE>(a, grid, s);
 }
 
 }  // namespace pcr
@@ -691,14 +725,16 @@ extern "C" int pcr_bounce_block(int stack, int block) {
     return 64;
 }
 extern "C" hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block,
-                                        hipStream_t s) {
+                                        int lds_scene, hipStream_t s) {
     using namespace pcr;
     block = pcr_bounce_block(stack, block);
-    if (stack <= 8) return launch_bounce_s<8>(*a, first != 0, grid, block, s);
-    if (stack <= 16) return launch_bounce_s<16>(*a, first != 0, grid, block, s);
-    if (stack <= 24) return launch_bounce_s<24>(*a, first != 0, grid, block, s);
-    if (stack <= 32) return launch_bounce_s<32>(*a, first != 0, grid, block, s);
-    return launch_bounce_s<64>(*a, first != 0, grid, block, s);
+    if (lds_scene && stack <= 8) return launch_bounce_s<8, true>(*a, first != 0, grid, block, s);
+    if (lds_scene && stack <= 16) return launch_bounce_s<16, true>(*a, first != 0, grid, block, s);
+    if (stack <= 8) return launch_bounce_s<8, false>(*a, first != 0, grid, block, s);
+    if (stack <= 16) return launch_bounce_s<16, false>(*a, first != 0, grid, block, s);
+    if (stack <= 24) return launch_bounce_s<24, false>(*a, first != 0, grid, block, s);
+    if (stack <= 32) return launch_bounce_s<32, false>(*a, first != 0, grid, block, s);
+    return launch_bounce_s<64, false>(*a, first != 0, grid, block, s);
 }
 extern "C" hipError_t pcr_launch_resolve(const pcr::ResolveArgs *a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(pcr::k_resolve, dim3(grid), dim3(256), 0, s, *a);
