@@ -73,6 +73,7 @@ struct DScene {
     const DLight *lights;
     const float4 *ltri;     // 4 x float4: v0|mat, e1|cum, e2|0, flatN|0
     int32_t n_sph, n_pl, n_tri, n_inner, n_leaves, n_mat, n_lights;
+    int32_t n_ltri, n_tabs;  // light triangles, table floats
     int32_t root_ref;       // >= 0 inner node, < 0 leaf ~index; valid if has_bvh
     int32_t has_bvh;
     float total_area;
@@ -107,11 +108,13 @@ __device__ __forceinline__ double ldg(const double *p) { return *(const PCR_GLOB
 // address space turns them into ds_read offsets).
 #define PCR_LDS __attribute__((address_space(3)))
 struct GeoGlobal {
+    static __device__ __forceinline__ float f1(const float *p) { return ldg(p); }
     static __device__ __forceinline__ float4 f4(const float4 *p) { return ldg(p); }
     static __device__ __forceinline__ int2 i2(const int2 *p) { return ldg(p); }
     static __device__ __forceinline__ int32_t i1(const int32_t *p) { return ldg(p); }
 };
 struct GeoLds {
+    static __device__ __forceinline__ float f1(const float *p) { return *(const PCR_LDS float *)(const void *)p; }
     static __device__ __forceinline__ float4 f4(const float4 *p) {
         const pcr_f4_t v = *(const PCR_LDS pcr_f4_t *)(const void *)p;
         return make_float4(v.x, v.y, v.z, v.w);
@@ -127,32 +130,34 @@ __device__ __forceinline__ void stg(float4 *p, float4 v) {
 }
 __device__ __forceinline__ void stg(float *p, float v) { *(PCR_GLOBAL float *)(void *)p = v; }
 __device__ __forceinline__ void stg(double *p, double v) { *(PCR_GLOBAL double *)(void *)p = v; }
-__device__ __forceinline__ DMat ldg(const DMat *p) {
+template <class GA>
+__device__ __forceinline__ DMat ld_mat(const DMat *p) {
     const float4 *q = (const float4 *)p;
     DMat m;
-    m.albedo = ldg(q);
-    m.emissive = ldg(q + 1);
-    m.fitA = ldg(q + 2);
-    m.fitE = ldg(q + 3);
-    const float4 e = ldg(q + 4);
+    m.albedo = GA::f4(q);
+    m.emissive = GA::f4(q + 1);
+    m.fitA = GA::f4(q + 2);
+    m.fitE = GA::f4(q + 3);
+    const float4 e = GA::f4(q + 4);
     m.flags = __float_as_uint(e.x);
     m.tab = __float_as_int(e.y);
     m.pad0 = m.pad1 = 0;
     m.pad2 = make_float4(0.f, 0.f, 0.f, 0.f);
     return m;
 }
-__device__ __forceinline__ DLight ldg(const DLight *p) {
+template <class GA>
+__device__ __forceinline__ DLight ld_light(const DLight *p) {
     const float4 *q = (const float4 *)p;
-    const float4 h = ldg(q);
+    const float4 h = GA::f4(q);
     DLight L;
     L.kind = __float_as_int(h.x);
     L.first = __float_as_int(h.y);
     L.count = __float_as_int(h.z);
     L.area = h.w;
-    L.o = ldg(q + 1);
-    L.u = ldg(q + 2);
-    L.v = ldg(q + 3);
-    L.n = ldg(q + 4);
+    L.o = GA::f4(q + 1);
+    L.u = GA::f4(q + 2);
+    L.v = GA::f4(q + 3);
+    L.n = GA::f4(q + 4);
     return L;
 }
 
@@ -262,13 +267,14 @@ __device__ __forceinline__ V3 fm_cmf(float l) {
 }
 
 // ------------------------------------------------------------- spectra -----
+template <class GA = GeoGlobal>
 __device__ __forceinline__ float spec_eval(const float *s, float l) {
     if (l < 400.f || l > 700.f) return 0.f;
     float t = div_rn(fsub(l, 400.f), 5.f);
     int i = (int)t;
-    if (i >= 60) return ldg(s + 60);
+    if (i >= 60) return GA::f1(s + 60);
     float f = fsub(t, (float)i);
-    return fadd(fmul(ldg(s + i), fsub(1.f, f)), fmul(ldg(s + i + 1), f));
+    return fadd(fmul(GA::f1(s + i), fsub(1.f, f)), fmul(GA::f1(s + i + 1), f));
 }
 __device__ __forceinline__ float sigmoid(float x) {
     return fadd(0.5f, div_rn(x, fmul(2.f, sqrt_rn(fadd(1.f, fmul(x, x))))));
@@ -278,12 +284,14 @@ __device__ __forceinline__ float eval_fit(float4 f, float l) {
     float p = fadd(f.x, fmul(L, fadd(f.y, fmul(L, f.z))));
     return fmul(sigmoid(p), f.w);
 }
+template <class GA = GeoGlobal>
 __device__ __forceinline__ float albedo_at(const DMat &m, const float *tabs, float l) {
-    if (m.flags & MF_TAB_ALBEDO) return spec_eval(tabs + m.tab, l);
+    if (m.flags & MF_TAB_ALBEDO) return spec_eval<GA>(tabs + m.tab, l);
     return smin(eval_fit(m.fitA, l), 1.f);
 }
+template <class GA = GeoGlobal>
 __device__ __forceinline__ float emissive_at(const DMat &m, const float *tabs, float l) {
-    if (m.flags & MF_TAB_EMISSIVE) return spec_eval(tabs + m.tab + 61, l);
+    if (m.flags & MF_TAB_EMISSIVE) return spec_eval<GA>(tabs + m.tab + 61, l);
     return eval_fit(m.fitE, l);
 }
 __device__ __forceinline__ float cauchy_ior(float ior, float B, float l) {
@@ -490,13 +498,13 @@ __device__ __forceinline__ void hit_geometry(const DScene &S, const HitRec &h, V
         const float4 *tn = S.tri_n + 4 * h.prim;
         if (__float_as_int(b.w)) {
             float w = fsub(fsub(1.f, h.u), h.v);
-            float4 n0 = ldg(tn), n1 = ldg(tn + 1), n2 = ldg(tn + 2);
+            float4 n0 = G::f4(tn), n1 = G::f4(tn + 1), n2 = G::f4(tn + 2);
             V3 in = v3(fadd(fadd(fmul(n0.x, w), fmul(n1.x, h.u)), fmul(n2.x, h.v)),
                        fadd(fadd(fmul(n0.y, w), fmul(n1.y, h.u)), fmul(n2.y, h.v)),
                        fadd(fadd(fmul(n0.z, w), fmul(n1.z, h.u)), fmul(n2.z, h.v)));
             N = normalize(in);
         } else {
-            N = xyz(ldg(tn + 3));
+            N = xyz(G::f4(tn + 3));
         }
     }
 }

@@ -60,25 +60,34 @@ struct PassInfo {
     int32_t n_rsegs;   // ResolveSeg entries
 };
 
-// A small scene's traversal data copied into LDS by every block of a bounce
-// launch (float4 units): spheres, sphere materials (4 per float4), plane
-// records, triangles, inner nodes, leaves (2 per float4).  Used when all jobs
-// of a pass share one scene that fits kGeoF4.
-constexpr int kGeoF4 = 512;
+// A small scene copied into LDS by every block of a bounce launch, in float4
+// units: spheres, sphere materials (4 per float4), plane records, triangles,
+// triangle normals, inner nodes, leaves (2 per float4), materials (6 per
+// material), lights (5 per light), light triangles, spectral tables (4 floats
+// per float4).  Used when all jobs of a pass share one scene that fits the
+// block's capacity, which keeps 3 waves per SIMD within the 160 KB of LDS next
+// to the traversal stacks.
 struct GeoLayout {
-    int sph, sphm, pl, tri, bvh, leaf, total;
+    int sph, sphm, pl, tri, trin, bvh, leaf, mat, light, ltri, tabs, total;
 };
-__host__ __device__ inline GeoLayout geo_layout(int n_sph, int n_pl, int n_tri, int n_inner, int n_leaves) {
+__host__ __device__ inline GeoLayout geo_layout(const DScene &S) {
     GeoLayout g;
     g.sph = 0;
-    g.sphm = n_sph;
-    g.pl = g.sphm + (n_sph + 3) / 4;
-    g.tri = g.pl + 4 * n_pl;
-    g.bvh = g.tri + 3 * n_tri;
-    g.leaf = g.bvh + 4 * n_inner;
-    g.total = g.leaf + (n_leaves + 1) / 2;
+    g.sphm = S.n_sph;
+    g.pl = g.sphm + (S.n_sph + 3) / 4;
+    g.tri = g.pl + 4 * S.n_pl;
+    g.trin = g.tri + 3 * S.n_tri;
+    g.bvh = g.trin + 4 * S.n_tri;
+    g.leaf = g.bvh + 4 * S.n_inner;
+    g.mat = g.leaf + (S.n_leaves + 1) / 2;
+    g.light = g.mat + (int)(sizeof(DMat) / 16) * S.n_mat;
+    g.ltri = g.light + (int)(sizeof(DLight) / 16) * S.n_lights;
+    g.tabs = g.ltri + 4 * S.n_ltri;
+    g.total = g.tabs + (S.n_tabs + 3) / 4;
     return g;
 }
+static_assert(sizeof(DMat) % 16 == 0 && sizeof(DLight) % 16 == 0, "LDS scene records are float4 multiples");
+__host__ __device__ constexpr int geo_cap(int block) { return block >= 256 ? 1024 : block >= 128 ? 512 : 256; }
 
 struct BounceArgs {
     const DJob *jobs;
@@ -113,7 +122,7 @@ struct IntersectArgs {
 
 extern "C" {
 // lds_scene: every job of the pass uses one scene whose geo_layout fits
-// kGeoF4 (stacks of at most 16 entries); the blocks read it from LDS.
+// geo_cap(block) (stacks of at most 16 entries); the blocks read it from LDS.
 hipError_t pcr_launch_bounce(const pcr::BounceArgs *a, int first, int stack, int grid, int block, int lds_scene,
                              hipStream_t s);
 // Threads per block the bounce launcher actually uses for a stack depth.

@@ -87,7 +87,7 @@ struct pcrhip_scene {
     int stack = 8;
     int depth = 0;
     int n_inner = 0, n_leaves = 0;
-    bool lds_scene = false;  // traversal data fit kGeoF4 and the stack is at most 16 entries
+    int geo_f4 = 0;  // float4s of the scene's LDS copy (geo_layout)
     uint64_t bytes = 0;
     std::atomic<int> jobs{0};
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // bounds of all geometry
@@ -432,7 +432,7 @@ int pass_class(const pcrhip_job *j) {
 // Launch shape of one pass.
 struct PassShape {
     int nb, stack, block, grid, rgrid;
-    bool lds_scene;  // the pass's one scene is read from LDS (geo_layout fits kGeoF4)
+    bool lds_scene;  // the pass's one scene is read from LDS (geo_layout fits geo_cap(block))
 };
 
 BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
@@ -646,10 +646,11 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     HIPCHK(hipMemcpyAsync(c->rsegs_dev, rsegs.data(), rsegs.size() * sizeof(ResolveSeg), hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipMemcpyAsync(c->pass_dev, c->pass_host, sizeof(PassInfo), hipMemcpyHostToDevice, c->stream));
-    bool one_scene = true;
-    for (auto *j : jobs) one_scene = one_scene && j->scene == jobs[0]->scene;
+    // one small scene: the bounce blocks read it from LDS
+    bool lds_scene = stack <= 16 && jobs[0]->scene->geo_f4 <= geo_cap(pcr_bounce_block(stack, cfg.block));
+    for (auto *j : jobs) lds_scene = lds_scene && j->scene == jobs[0]->scene;
     const PassShape sh{std::max(max_depth, 1), stack, cfg.block, c->n_cu * std::max(1, cfg.blocks_per_cu),
-                       c->n_cu * 4, one_scene && jobs[0]->scene->lds_scene};
+                       c->n_cu * 4, lds_scene};
     const int nb = sh.nb;
     // Per-kernel event timing needs the launches on the stream itself, so a
     // profiled pass of the graph schedule is issued as the same persistent
@@ -995,7 +996,9 @@ int build_scene(pcrhip_scene *s, const pcrhip_scene_desc *d) {
     S.n_mat = d->n_materials;
     S.n_lights = d->n_lights;
     S.total_area = total;
-    s->lds_scene = s->stack <= 16 && geo_layout(S.n_sph, S.n_pl, S.n_tri, S.n_inner, S.n_leaves).total <= kGeoF4;
+    S.n_ltri = d->n_light_tris;
+    S.n_tabs = (int32_t)tabs.size();
+    s->geo_f4 = geo_layout(S).total;
     S.cam = make_float3(d->cam_pos[0], d->cam_pos[1], d->cam_pos[2]);
     S.fov = d->cam_fov;
     // bounds of all geometry (input normalisation of the guiding network)

@@ -193,11 +193,11 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
     hit_geometry<GA>(S, h, ps.o, ps.d, hit, N, mi);
     bool entering = dot(ps.d, N) < 0.f;
     if (!entering) N = mul(N, -1.f);
-    const DMat M = ldg(S.mat + mi);
+    const DMat M = ld_mat<GA>(S.mat + mi);
     if (M.flags & MF_EMISSIVE) {
         cnt.emit++;
         _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) {
-            float le = spectral ? emissive_at(M, S.tabs, lam[c]) : (c == 0 ? M.emissive.x : (c == 1 ? M.emissive.y : M.emissive.z));
+            float le = spectral ? emissive_at<GA>(M, S.tabs, lam[c]) : (c == 0 ? M.emissive.x : (c == 1 ? M.emissive.y : M.emissive.z));
             ps.rad[c] = fadd(ps.rad[c], fmul(ps.tp[c], le));
         }
         write_radiance(J, ps);
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
     }
     float alb[4] = {0.f, 0.f, 0.f, 0.f};
     _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch)
-        alb[c] = spectral ? albedo_at(M, S.tabs, lam[c]) : (c == 0 ? M.albedo.x : (c == 1 ? M.albedo.y : M.albedo.z));
+        alb[c] = spectral ? albedo_at<GA>(M, S.tabs, lam[c]) : (c == 0 ? M.albedo.x : (c == 1 ? M.albedo.y : M.albedo.z));
 
     const int nb = bounce + 1;
     if (M.flags & MF_METALLIC) {
@@ -288,11 +288,11 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
                 float target = fmul(rng_next(ps.rng), S.total_area);
                 float cum = 0.f;
                 for (int k = 0; k < S.n_lights; k++) {
-                    cum = fadd(cum, ldg(&S.lights[k].area));
+                    cum = fadd(cum, GA::f1(&S.lights[k].area));
                     if (target <= cum) { li = k; break; }
                 }
             }
-            const DLight L = ldg(S.lights + li);
+            const DLight L = ld_light<GA>(S.lights + li);
             V3 sP, sN;
             int lmat;
             if (L.kind == 0) {
@@ -305,7 +305,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
                 int lo = 0, hi = L.count;
                 while (lo < hi) {  // std::lower_bound on the cumulative areas
                     int mid = (lo + hi) >> 1;
-                    if (ldg(S.ltri + 4 * (L.first + mid) + 1).w < rt) lo = mid + 1;
+                    if (GA::f4(S.ltri + 4 * (L.first + mid) + 1).w < rt) lo = mid + 1;
                     else hi = mid;
                 }
                 int ti = L.first + (lo < L.count - 1 ? lo : L.count - 1);
@@ -315,7 +315,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
                     r2 = fsub(1.f, r2);
                 }
                 const float4 *lt = S.ltri + 4 * ti;
-                float4 a = ldg(lt), e1 = ldg(lt + 1), e2 = ldg(lt + 2), fn = ldg(lt + 3);
+                float4 a = GA::f4(lt), e1 = GA::f4(lt + 1), e2 = GA::f4(lt + 2), fn = GA::f4(lt + 3);
                 sP = add(add(xyz(a), mul(xyz(e1), r1)), mul(xyz(e2), r2));
                 sN = xyz(fn);
                 lmat = __float_as_int(a.w);
@@ -341,7 +341,8 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
                                : sh.kind == 1 ? __float_as_int(GA::f4(S.pl + 4 * sh.prim + 2).w)
                                               : __float_as_int(GA::f4(S.tri + 3 * sh.prim).w);
                 const V3 dd = sub(add(so, mul(wi, sh.t)), so);
-                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) && !(ldg(&S.mat[sm].flags) & MF_EMISSIVE);
+                occl = (double)d2 - 1e-3 > (double)dot(dd, dd) &&
+                       !((uint32_t)GA::i1((const int32_t *)&S.mat[sm].flags) & MF_EMISSIVE);
             }
             if (!occl) {
                 float cosL = smax(0.f, dot(sN, mul(Li, -1.f)));
@@ -353,9 +354,9 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
                     w = div_rn(fmul(pl, pl), fadd(fmul(pl, pl), fmul(pb, pb)));
                 }
                 if (lmat != le_mat) {
-                    const DMat LM = ldg(S.mat + lmat);
+                    const DMat LM = ld_mat<GA>(S.mat + lmat);
                     _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch)
-                        le_c[c] = spectral ? emissive_at(LM, S.tabs, lam[c])
+                        le_c[c] = spectral ? emissive_at<GA>(LM, S.tabs, lam[c])
                                            : (c == 0 ? LM.emissive.x : (c == 1 ? LM.emissive.y : LM.emissive.z));
                     le_mat = lmat;
                 }
@@ -457,9 +458,9 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // waves (128 VGPRs, ~200 bytes of spills) slower.  Up to 24 stack entries,
 // 3 waves per SIMD also fit LDS (12 waves x 64 lanes x 24 x 8 bytes = 147 KB
 // of 160 KB); deeper stacks are limited by LDS and keep the compiler's choice.
-// LDS_SCENE: every job of the pass shares one scene small enough for kGeoF4;
-// each block copies its traversal data (geo_layout) into LDS first and the
-// traversal reads them there.
+// LDS_SCENE: every job of the pass shares one scene small enough for
+// geo_cap(BLOCK); each block copies it (geo_layout) into LDS first, and the
+// traversal, materials, lights and spectral tables are read there.
 template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 24 ? 3 : 1, STACK <= 24 ? 3 : 10)))
 void k_bounce(BounceArgs a) {
@@ -468,21 +469,29 @@ void k_bounce(BounceArgs a) {
     __shared__ uint32_t s_base;
     __shared__ uint32_t s_pref[kMaxSegs + 1];
     __shared__ GenSeg s_segs[kMaxSegs];
-    __shared__ float4 s_geo[LDS_SCENE ? kGeoF4 : 1];
+    __shared__ float4 s_geo[LDS_SCENE ? geo_cap(BLOCK) : 1];
     using GA = typename std::conditional<LDS_SCENE, GeoLds, GeoGlobal>::type;
     LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
     const int n_segs = a.pass->n_segs;
     const uint32_t lane = lane_id();
     GeoLayout gl{};
-    if (LDS_SCENE) {
+    if (LDS_SCENE) {  // the host checked geo_layout(S0).total <= geo_cap(BLOCK)
         const DScene &S0 = a.jobs[a.segs[0].slot].S;
-        gl = geo_layout(S0.n_sph, S0.n_pl, S0.n_tri, S0.n_inner, S0.n_leaves);
+        gl = geo_layout(S0);
+        auto copy4 = [&](int off, const void *src, int n4) {
+            for (int i = threadIdx.x; i < n4; i += BLOCK) s_geo[off + i] = ldg((const float4 *)src + i);
+        };
         for (int i = threadIdx.x; i < S0.n_sph; i += BLOCK) s_geo[gl.sph + i] = ldg(S0.sph + i);
         for (int i = threadIdx.x; i < S0.n_sph; i += BLOCK) ((int32_t *)(s_geo + gl.sphm))[i] = ldg(S0.sph_mat + i);
-        for (int i = threadIdx.x; i < 4 * S0.n_pl; i += BLOCK) s_geo[gl.pl + i] = ldg(S0.pl + i);
-        for (int i = threadIdx.x; i < 3 * S0.n_tri; i += BLOCK) s_geo[gl.tri + i] = ldg(S0.tri + i);
-        for (int i = threadIdx.x; i < 4 * S0.n_inner; i += BLOCK) s_geo[gl.bvh + i] = ldg(S0.bvh + i);
+        copy4(gl.pl, S0.pl, 4 * S0.n_pl);
+        copy4(gl.tri, S0.tri, 3 * S0.n_tri);
+        copy4(gl.trin, S0.tri_n, 4 * S0.n_tri);
+        copy4(gl.bvh, S0.bvh, 4 * S0.n_inner);
         for (int i = threadIdx.x; i < S0.n_leaves; i += BLOCK) ((int2 *)(s_geo + gl.leaf))[i] = ldg(S0.leaves + i);
+        copy4(gl.mat, S0.mat, (int)(sizeof(DMat) / 16) * S0.n_mat);
+        copy4(gl.light, S0.lights, (int)(sizeof(DLight) / 16) * S0.n_lights);
+        copy4(gl.ltri, S0.ltri, 4 * S0.n_ltri);
+        for (int i = threadIdx.x; i < S0.n_tabs; i += BLOCK) ((float *)(s_geo + gl.tabs))[i] = ldg(S0.tabs + i);
     }
     // The segment table and the exclusive prefix of the segments' input
     // counts, staged in LDS (one wave scans <= 64 segments): every ray's queue
@@ -559,8 +568,13 @@ void k_bounce(BounceArgs a) {
                     Sl.sph_mat = (const int32_t *)(s_geo + gl.sphm);
                     Sl.pl = s_geo + gl.pl;
                     Sl.tri = s_geo + gl.tri;
+                    Sl.tri_n = s_geo + gl.trin;
                     Sl.bvh = s_geo + gl.bvh;
                     Sl.leaves = (const int2 *)(s_geo + gl.leaf);
+                    Sl.mat = (const DMat *)(s_geo + gl.mat);
+                    Sl.lights = (const DLight *)(s_geo + gl.light);
+                    Sl.ltri = s_geo + gl.ltri;
+                    Sl.tabs = (const float *)(s_geo + gl.tabs);
                     alive = bounce_path<STACK, GA>(J, Sl, ps, st, cnt, fork);
                 } else {
                     alive = bounce_path<STACK, GA>(J, J.S, ps, st, cnt, fork);
