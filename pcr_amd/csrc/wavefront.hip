@@ -450,7 +450,7 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // region [4*begin_k, 4*(begin_k + count_k)) of both ping-pong queues -- room
 // for every path of the job to fork into its 4 hero channels -- and a live-ray
 // counter per bounce.  A launch walks the concatenation of the segments' live
-// rays in 128-ray chunks, so a wave almost always holds rays of one job: the
+// rays in 128-ray chunks per wave, so a wave almost always holds rays of one job: the
 // job's descriptor and scene are read with scalar loads and the shading code
 // runs once per wave.
 // Occupancy: 3 waves per SIMD (<= 168 VGPRs, a few dozen bytes of spills)
@@ -466,7 +466,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 
 void k_bounce(BounceArgs a) {
     __shared__ int32_t s_idx[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
-    __shared__ uint32_t s_base;
     __shared__ uint32_t s_pref[kMaxSegs + 1];
     __shared__ GenSeg s_segs[kMaxSegs];
     __shared__ float4 s_geo[LDS_SCENE ? geo_cap(BLOCK) : 1];
@@ -517,13 +516,22 @@ void k_bounce(BounceArgs a) {
     const uint32_t n_in = s_pref[kMaxSegs];
     LaneCounters lc;
     int wslot = -1;  // job slot of the counts in lc (wave-uniform)
+    // Each wave claims its own chunks of 128 rays and takes them 64 at a time:
+    // no block barrier per chunk, so a wave with long paths (forks, NEE) does
+    // not hold up the others, and one atomic serves two rounds.
+    uint32_t chunk = 0, rounds = 0;  // wave-uniform
     for (;;) {
-        if (threadIdx.x == 0) s_base = atomicAdd(a.work, (uint32_t)BLOCK);
-        __syncthreads();
-        const uint32_t base = s_base;
-        __syncthreads();
+        if (rounds == 0) {
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(a.work, 128u);
+            chunk = __builtin_amdgcn_readfirstlane(c);
+            rounds = 2;
+        }
+        const uint32_t base = chunk;
+        chunk += 64u;
+        rounds--;
         if (base >= n_in) break;
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t i = base + lane;
         PathState ps;
         bool ok = false;
         int seg = -1;
