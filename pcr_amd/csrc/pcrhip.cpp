@@ -88,6 +88,7 @@ struct pcrhip_scene {
     int depth = 0;
     int n_inner = 0, n_leaves = 0;
     int geo_f4 = 0;  // float4s of the scene's LDS copy (geo_layout)
+    int id = 0;      // distinct per context
     uint64_t bytes = 0;
     std::atomic<int> jobs{0};
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // bounds of all geometry
@@ -160,7 +161,8 @@ struct pcrhip_ctx {
     pcrhip_job *active[kSlots] = {};
     uint64_t mem_in_use = 0;
     std::string fatal;
-    uint64_t pass_turn = 0;  // rotates passes between traversal-stack classes
+    uint64_t pass_turn = 0;  // rotates passes between pass classes
+    std::atomic<int> next_scene_id{0};
     // profiling
     std::atomic<bool> profile{false};
     std::mutex pm;
@@ -423,10 +425,18 @@ int stack_class(int stack) {
     return stack <= 8 ? 8 : stack <= 16 ? 16 : stack <= 24 ? 24 : stack <= 32 ? 32 : 64;
 }
 
-// Jobs that may share a pass: same stack variant; a guided job trains its own
-// network on its pass's paths, so it runs alone.
-int pass_class(const pcrhip_job *j) {
-    return j->guided ? 1000 + j->slot : stack_class(j->scene->stack);
+// Jobs that may share a pass.  A scene small enough for the bounce kernel's
+// LDS copy gets passes of its own, which read it from LDS.  The other stack
+// variants up to 24 entries all run at 3 waves per SIMD, so their jobs share
+// passes (the pass takes the largest stack); deeper stacks cost occupancy and
+// keep passes of their own.  A guided job trains its own network on its
+// pass's paths, so it runs alone.
+int pass_class(const pcrhip_job *j, int block) {
+    if (j->guided) return 1000000 + j->slot;
+    const pcrhip_scene *sc = j->scene;
+    if (sc->stack <= 16 && sc->geo_f4 <= geo_cap(pcr_bounce_block(sc->stack, block))) return 100000 + sc->id;
+    const int c = stack_class(sc->stack);
+    return c <= 24 ? 24 : c;
 }
 
 // Launch shape of one pass.
@@ -748,19 +758,20 @@ void worker_main(pcrhip_ctx *c) {
                 else run.push_back(j);
             }
         if (run.empty()) continue;
-        // One traversal-stack class per pass: the bounce kernel's LDS stack is
-        // sized for the deepest BVH in the pass, so a deep scene in the mix
-        // would cost every other job occupancy.  Classes take turns; a guided
-        // job is a class of its own.
+        // One pass class per pass (pass_class): a small scene alone, so the
+        // bounce kernel reads it from LDS; otherwise one traversal-stack range,
+        // because a deeper stack would cost every other job occupancy.
+        // Classes take turns; a guided job is a class of its own.
         {
             std::vector<int> classes;
-            for (auto *j : run) classes.push_back(pass_class(j));
+            const int blk = c->cfg.block;
+            for (auto *j : run) classes.push_back(pass_class(j, blk));
             std::sort(classes.begin(), classes.end());
             classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
             if (classes.size() > 1) {
                 const int cls = classes[c->pass_turn++ % classes.size()];
                 run.erase(std::remove_if(run.begin(), run.end(),
-                                         [cls](const pcrhip_job *j) { return pass_class(j) != cls; }),
+                                         [cls, blk](const pcrhip_job *j) { return pass_class(j, blk) != cls; }),
                           run.end());
             }
         }
@@ -1149,6 +1160,7 @@ int pcrhip_scene_upload(pcrhip_ctx *c, const pcrhip_scene_desc *d, pcrhip_scene 
     HIPCHK(hipSetDevice(c->device));
     auto *s = new pcrhip_scene();
     s->ctx = c;
+    s->id = c->next_scene_id++;
     int rc = build_scene(s, d);
     if (rc) {
         free_scene(s);
