@@ -33,6 +33,8 @@ constexpr int kGMaxSteps = 128;  // training steps per pass (at most)
 // Training defaults per pass: Adam steps (the records are split evenly over
 // them) and vertices recorded.
 constexpr int kGDefaultSteps = 128, kGDefaultRecords = 262144;
+// Largest guided pass (paths): training runs after every pass.
+constexpr uint64_t kGuidedPassPaths = 16ull << 20;
 
 // The inference kernels keep a wave's activations in registers: rays are the
 // N dimension of v_mfma_f32_16x16x32_bf16 (H^T = W^T X^T), so a layer's

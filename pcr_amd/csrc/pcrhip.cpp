@@ -589,9 +589,10 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         uint64_t left = std::max<uint64_t>(cfg.target_pass_paths, 1);
         if (guided) {
             // online training: the first passes are small (1, 2, 4, ... samples
-            // per pixel) so the network learns early in the job
+            // per pixel) so the network learns early in the job, and passes
+            // stay at most 16 M paths, since the network trains once per pass
             const uint64_t pix = (uint64_t)jobs[0]->cw * jobs[0]->ch;
-            left = std::min<uint64_t>(left, pix << std::min(jobs[0]->passes, 20));
+            left = std::min<uint64_t>({left, pix << std::min(jobs[0]->passes, 20), kGuidedPassPaths});
         }
         std::vector<size_t> open(nj);
         for (size_t i = 0; i < nj; i++) open[i] = i;
@@ -1070,7 +1071,10 @@ int pcrhip_create(int device, pcrhip_ctx **out) {
     c->n_cu = prop.multiProcessorCount;
     c->cfg.block = 128;
     c->cfg.blocks_per_cu = 8;
-    c->cfg.target_pass_paths = 16u << 20;
+    // 64 M paths per pass: a 720x720x64 frame is one pass, and fewer, larger
+    // passes pay the bounce launches' drain tails less often (queue memory
+    // 4 x 2 x 80 B per path: about 43 GB of the 288 GB).
+    c->cfg.target_pass_paths = 64u << 20;
     c->cfg.memory_budget = 64ull << 30;
     c->cfg.max_active_jobs = 32;
     c->cfg.schedule = PCRHIP_SCHED_GRAPH;
@@ -1428,14 +1432,22 @@ int pcrhip_autotune(pcrhip_ctx *c, pcrhip_scene *s, const pcrhip_job_desc *d, ui
         cfg.schedule = sched;
         rc = consider(cfg);
     }
-    // stage 3: pass size (queue memory), up to twice the trial's paths
+    // stage 3: a smaller pass (less queue memory), up to twice the trial's
+    // paths, only if it is at least 2 % faster: passes at least as large as
+    // the trial all run it in one pass, and noise must not pick a size that
+    // splits the real job.
     const uint64_t tp = pix * (uint64_t)t.spp;
-    for (uint64_t pp : {2ull << 20, 4ull << 20, 8ull << 20, 16ull << 20, 32ull << 20}) {
+    for (uint64_t pp : {4ull << 20, 8ull << 20, 16ull << 20, 32ull << 20}) {
         if (rc) break;
-        if (pp == best.target_pass_paths || pp > 2 * tp) continue;
+        if (pp >= best.target_pass_paths || pp > 2 * tp) continue;
         pcrhip_config cfg = best;
         cfg.target_pass_paths = pp;
-        rc = consider(cfg);
+        double ms = 0;
+        if ((rc = trial(cfg, ms))) break;
+        if (ms < 0.98 * best_ms) {
+            best_ms = ms;
+            best = cfg;
+        }
     }
     std::string err = rc ? g_err : std::string();
     (void)pcrhip_set_config(c, rc ? &base : &best);
