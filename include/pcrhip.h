@@ -191,10 +191,13 @@ typedef struct {
     int priority;        /* higher is admitted first */
     /* PCRHIP_GUIDED training per pass (0 selects the default): Adam steps
      * (<= 128, default 128), learning rate (default 5e-3), training samples
-     * recorded (default 262144). */
+     * recorded (default 262144).  guide_bounces: the vertices of bounce index
+     * below it draw from the learned mixture, later ones sample the cosine
+     * density alone (default 3). */
     int guide_steps;
     float guide_lr;
     int guide_records;
+    int guide_bounces;
 } pcrhip_job_desc;
 
 /* Queues a job on the context's serving thread (continuous batching: the job

@@ -33,6 +33,10 @@ constexpr int kGMaxSteps = 128;  // training steps per pass (at most)
 // Training defaults per pass: Adam steps (the records are split evenly over
 // them) and vertices recorded.
 constexpr int kGDefaultSteps = 128, kGDefaultRecords = 262144;
+// Guided vertices per path by default (bounce index 0, 1, 2): later vertices
+// carry little of the variance (their throughput has fallen by the albedos)
+// and keep cosine sampling, which spares their inference.
+constexpr int kGDefaultBounces = 3;
 // Largest guided pass (paths): training runs after every pass.
 constexpr uint64_t kGuidedPassPaths = 16ull << 20;
 

@@ -21,6 +21,7 @@ struct DJob {
     int32_t strata;          // round(sqrt(spp)) when stratified
     uint64_t seed;
     float scale, aspect;     // tan(fov/2) (host libm) and W/H
+    int32_t guide_bounces;   // guided jobs: vertices (bounce index below it) drawn from the learned mixture
     int32_t spp_pass;        // samples per pixel in one pass
     int32_t s_begin;         // first sample index of the current pass
     float4 *radiance;        // [cw*ch*spp_pass] channel radiance of each path

@@ -220,7 +220,8 @@ int validate_job(const pcrhip_scene *s, const pcrhip_job_desc *d) {
         return fail(PCRHIP_EINVAL, "too many samples");
     if (s->S.n_lights < 1) return fail(PCRHIP_EINVAL, "scene has no light");
     if (d->guide_steps < 0 || d->guide_steps > kGMaxSteps) return fail(PCRHIP_EINVAL, "guide_steps must be in 0..128");
-    if (!(d->guide_lr >= 0.f) || d->guide_records < 0) return fail(PCRHIP_EINVAL, "bad guide_lr or guide_records");
+    if (!(d->guide_lr >= 0.f) || d->guide_records < 0 || d->guide_bounces < 0)
+        return fail(PCRHIP_EINVAL, "bad guide_lr, guide_records or guide_bounces");
     return PCRHIP_OK;
 }
 
@@ -243,6 +244,7 @@ void fill_djob(const pcrhip_job *j, DJob &J) {
     // Renderer.cpp:149-150 (host libm tanf, as the reference)
     J.aspect = d.width / (float)d.height;
     J.scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * j->scene->S.fov);
+    J.guide_bounces = d.guide_bounces > 0 ? d.guide_bounces : kGDefaultBounces;
     J.spp_pass = j->spp_pass;
     J.s_begin = j->s_next;
     J.radiance = nullptr;  // set per pass: a window of the pass radiance buffer
@@ -520,6 +522,9 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
         HIPCHK(pcr_launch_bounce(&a, b == 0, sh.stack, sh.grid, sh.block, sh.lds_scene, c->stream));
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b + 1], c->stream));
+        // rays leaving launch b carry bounce index b: only the guided ones pend
+        const int gb = j->d.guide_bounces > 0 ? j->d.guide_bounces : kGDefaultBounces;
+        if (b >= gb) continue;
         GuideArgs ga{g, a.out, a.count_out, 4u * segs[0].count, c->grec, rec_cap, keep, j->adam_t > 0};
         HIPCHK(pcr_launch_guide(&ga, c->n_cu, c->stream));
     }

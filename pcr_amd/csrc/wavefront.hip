@@ -375,7 +375,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
         }
         _Pragma("unroll") for (int c = 0; c < 4; c++) if (c < nch) ps.tp[c] = div_rn(ps.tp[c], p);
     }
-    if (J.flags & JF_GUIDED) {
+    if ((J.flags & JF_GUIDED) && bounce < J.guide_bounces) {
         // The guiding kernel draws the direction after this launch from the
         // MIS mixture of cosine and learned lobes; d carries the normal.
         ps.d = N;
