@@ -167,6 +167,10 @@ int pcrhip_scene_release(pcrhip_scene *scene);
 typedef struct {
     int n_inner_nodes, n_leaves, bvh_depth, traversal_stack;
     uint64_t device_bytes;
+    /* bytes of the copy a bounce block keeps in LDS when a pass renders only
+     * this scene (used if it fits the block's share: 16 KB at 256 threads,
+     * 8 KB at 128, 4 KB at 64, and the traversal stack is at most 16) */
+    uint64_t lds_bytes;
 } pcrhip_scene_info;
 int pcrhip_scene_get_info(pcrhip_scene *scene, pcrhip_scene_info *out);
 

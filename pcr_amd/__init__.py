@@ -59,7 +59,7 @@ class TuneEntry(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("n_inner_nodes", C.c_int), ("n_leaves", C.c_int), ("bvh_depth", C.c_int),
-                ("traversal_stack", C.c_int), ("device_bytes", C.c_uint64)]
+                ("traversal_stack", C.c_int), ("device_bytes", C.c_uint64), ("lds_bytes", C.c_uint64)]
 
 
 class JobDesc(C.Structure):

@@ -1196,6 +1196,7 @@ int pcrhip_scene_get_info(pcrhip_scene *s, pcrhip_scene_info *out) {
     out->bvh_depth = s->depth;
     out->traversal_stack = s->stack;
     out->device_bytes = s->bytes;
+    out->lds_bytes = 16ull * (uint64_t)s->geo_f4;
     return PCRHIP_OK;
 }
 

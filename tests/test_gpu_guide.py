@@ -8,7 +8,8 @@
     backpropagation: relative L2 error < 2e-2, cosine > 0.999 -- the oracle's
     own gradient is checked against central differences on the CPU;
   * a guided render is unbiased: per-channel image means within 3 standard
-    errors of the plain render's (cosine sampling keeps half the mixture);
+    errors of the plain render's (cosine sampling keeps half the mixture),
+    with the default 3 guided vertices per path and with every vertex guided;
   * training lowers the loss and the guided estimator beats plain sampling at
     equal samples on the closed box with the small light (relMSE against a
     high-sample plain render).
@@ -74,14 +75,15 @@ def render_mean(ds, seed, **kw):
     return s / kw["spp"], losses
 
 
-def test_guided_render_unbiased(gpu_ctx):
+@pytest.mark.parametrize("bounces", [0, 8])  # the default (3 guided vertices per path), every vertex
+def test_guided_render_unbiased(gpu_ctx, bounces):
     ds = gpu_ctx.upload(pcr_amd.HostScene(os.path.join(SC, "closed-c4.json")))
     try:
         kw = dict(width=96, height=96, depth=8, spp=32, shadow=4)
         K = 8
         plain = np.array([render_mean(ds, ol.spread_seed(40 + k), **kw)[0].reshape(-1, 3).mean(0) for k in range(K)])
-        guided = np.array([render_mean(ds, ol.spread_seed(80 + k), guided=True, **kw)[0].reshape(-1, 3).mean(0)
-                           for k in range(K)])
+        guided = np.array([render_mean(ds, ol.spread_seed(80 + k), guided=True, guide_bounces=bounces,
+                                       **kw)[0].reshape(-1, 3).mean(0) for k in range(K)])
         se = np.sqrt(plain.var(0, ddof=1) / K + guided.var(0, ddof=1) / K)
         z = (guided.mean(0) - plain.mean(0)) / se
         print("plain", plain.mean(0), "guided", guided.mean(0), "z", z)

@@ -62,6 +62,29 @@ def test_schedules_bit_identical(ctx_cfg, name, kw):
         ds.release()
 
 
+@pytest.mark.parametrize("name,kw", [
+    ("cornell-c1", dict(width=40, height=40, depth=8, spp=6, shadow=4)),
+    ("cornell-c0", dict(width=32, height=32, depth=4, spp=4, shadow=4, stratified=True)),
+])
+def test_scene_in_lds_bit_identical(ctx_cfg, name, kw):
+    """A pass with one small scene reads it from an LDS copy when it fits the
+    block's share (16 KB at 256 threads, 4 KB at 64); both paths must equal
+    the oracle bit for bit."""
+    path = os.path.join(SC, name + ".json")
+    ds = ctx_cfg.upload(pcr_amd.HostScene(path))
+    try:
+        lds = ds.info().lds_bytes
+        assert 4096 < lds <= 16384, lds  # block 64: global memory; block 256: LDS
+        want, _ = oracle(path).render_paths(kw["width"], kw["height"], kw["depth"], kw["spp"], kw["shadow"], 777,
+                                            **{k: v for k, v in kw.items() if k in ("stratified",)})
+        for block in (64, 256):
+            ctx_cfg.set_config(block=block)
+            got, _ = render_sums(ds, seed=777, **kw)
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), block
+    finally:
+        ds.release()
+
+
 def test_graph_mixed_jobs(ctx_cfg):
     """Jobs of different scenes and depths share replayed pass graphs."""
     ctx_cfg.set_config(schedule=pcr_amd.SCHED_GRAPH, target_pass_paths=1024)
