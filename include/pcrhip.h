@@ -51,7 +51,15 @@ enum {
      * own paths proposes an 8-lobe vMF mixture, MIS-combined 50/50 with cosine
      * sampling (no counterpart in the reference, whose diffuse bounce is pure
      * cosine sampling, Renderer.cpp:526-557). */
-    PCRHIP_GUIDED = 1u << 5
+    PCRHIP_GUIDED = 1u << 5,
+    /* Adaptive sampling (useAdaptive, Renderer.cpp:266-294; per-path form of
+     * path_trace_pass_adaptive, MetalRenderer.mm:1368-1552): the samples of a
+     * pixel form groups of adaptive_group (the reference's AA iterations of
+     * `samples` paths); each group's mean is one point of a float Welford
+     * mean/variance, and once at least 4 points have every channel's
+     * variance / (mean^2 + 0.01) below 0.05 the pixel takes no more samples.
+     * The result is the Welford mean. */
+    PCRHIP_ADAPTIVE = 1u << 6
 };
 
 /* Material flag bits (pcrhip_scene_desc.mat_flags), Material.h:14-62 */
@@ -202,6 +210,8 @@ typedef struct {
     float guide_lr;
     int guide_records;
     int guide_bounces;
+    /* PCRHIP_ADAPTIVE: samples per Welford point (spp must be a multiple) */
+    int adaptive_group;
 } pcrhip_job_desc;
 
 /* Queues a job on the context's serving thread (continuous batching: the job
@@ -223,9 +233,12 @@ typedef struct {
     int passes;
 } pcrhip_stats;
 /* Per-pixel MEAN over spp of linear XYZ (spectral) or linear RGB, cw*ch*3
- * floats, row-major over the crop window.  stats may be NULL. */
+ * floats, row-major over the crop window (PCRHIP_ADAPTIVE: the Welford mean
+ * over the samples taken).  stats may be NULL; stats.paths counts the paths
+ * traced. */
 int pcrhip_job_result(pcrhip_job *job, float *out, size_t n_floats, pcrhip_stats *stats);
-/* The raw FP64 per-pixel SUMS over samples (cw*ch*3 doubles). */
+/* The raw FP64 per-pixel SUMS over samples (cw*ch*3 doubles); for
+ * PCRHIP_ADAPTIVE jobs, the Welford means instead. */
 int pcrhip_job_result_sums(pcrhip_job *job, double *out, size_t n_doubles);
 int pcrhip_job_release(pcrhip_job *job);
 

@@ -1615,6 +1615,72 @@ int orc_render_paths(void *sp, const orc_path_params *pp, double *accum, uint64_
     return 0;
 }
 
+// Adaptive sampling (per-path form): Renderer.cpp:266-294 for the Welford
+// update and the stop rule, MetalRenderer.mm:1500-1540 for a point being the
+// mean of one AA iteration of `group` paths.
+int orc_render_paths_adaptive(void *sp, const orc_path_params *pp, int group, float *mean_out, uint64_t *counters) {
+    Scene &S = *(Scene *)sp;
+    const orc_path_params &P = *pp;
+    if (!S.finalized || S.lights.empty()) return -1;
+    if (P.x0 < 0 || P.y0 < 0 || P.x0 + P.cw > P.width || P.y0 + P.ch > P.height) return -2;
+    if (group < 1 || P.s_begin != 0 || P.s_end != P.spp || P.spp % group != 0) return -3;
+    int nth = P.threads > 0 ? P.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    float aspect = P.width / (float)P.height;
+    float scale = std::tan((float)std::numbers::pi / 180.f * 0.5f * S.fov);
+    std::vector<std::array<uint64_t, 4>> cnt(nth, std::array<uint64_t, 4>{0, 0, 0, 0});
+    std::atomic<int> nextRow{0};
+    auto worker = [&](int tid) {
+        PathCtx C{S, P, scale, aspect};
+        uint64_t paths = 0;
+        for (;;) {
+            int r = nextRow.fetch_add(1);
+            if (r >= P.ch) break;
+            int py = P.y0 + r;
+            for (int cx = 0; cx < P.cw; cx++) {
+                int px = P.x0 + cx;
+                double a[3] = {0, 0, 0};
+                float mean[3] = {0.f, 0.f, 0.f}, m2[3] = {0.f, 0.f, 0.f};
+                unsigned points = 0;
+                for (int s = 0; s < P.spp; s++) {
+                    float rad[4], o3[3];
+                    path_sample(C, px, py, s, rad, o3);
+                    for (int c = 0; c < 3; c++) a[c] += (double)o3[c];
+                    paths++;
+                    if ((s + 1) % group != 0) continue;
+                    points++;
+                    bool conv = points >= 4;
+                    for (int c = 0; c < 3; c++) {
+                        const float m = (float)(a[c] / (double)group);
+                        const float d = m - mean[c];
+                        mean[c] = mean[c] + d / (float)points;
+                        m2[c] = m2[c] + d * (m - mean[c]);
+                        if (points >= 4) {
+                            const float v = m2[c] / (float)(points - 1);
+                            conv = conv && v / (mean[c] * mean[c] + 0.01f) < 0.05f;
+                        }
+                        a[c] = 0;
+                    }
+                    if (conv) break;
+                }
+                size_t idx = (size_t)r * P.cw + cx;
+                for (int c = 0; c < 3; c++) mean_out[3 * idx + c] = mean[c];
+            }
+        }
+        cnt[tid] = {paths, C.n_ext, C.n_shadow, C.n_emit};
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++) th.emplace_back(worker, t);
+    for (auto &t : th) t.join();
+    if (counters) {
+        for (int k = 0; k < 4; k++) {
+            uint64_t v = 0;
+            for (int t = 0; t < nth; t++) v += cnt[t][k];
+            counters[k] = v;
+        }
+    }
+    return 0;
+}
+
 int orc_trace_one(void *sp, const orc_path_params *pp, int px, int py, int sample, float rad[4], float out3[3]) {
     Scene &S = *(Scene *)sp;
     const orc_path_params &P = *pp;

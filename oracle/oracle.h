@@ -90,6 +90,15 @@ typedef struct {
 /* accum: cw*ch*3 doubles, SUM over samples of per-path XYZ (spectral) or RGB.
  * counters[4]: paths, extension rays, shadow rays, emissive hits. */
 int orc_render_paths(void *s, const orc_path_params *p, double *accum, uint64_t *counters);
+/* Adaptive sampling with the per-path estimator: each pixel's samples form
+ * groups of `group` (the reference GPU backend's AA iterations,
+ * MetalRenderer.mm:1368-1552); a group's mean (FP64 sum / group, rounded to
+ * float) is one point of a float Welford mean/M2 (Renderer.cpp:266-294), and
+ * the pixel stops once it has >= 4 points with every channel's
+ * (M2 / (points - 1)) / (mean^2 + 0.01) below 0.05.  mean: cw*ch*3 floats.
+ * Needs s_begin = 0, s_end = spp and spp a multiple of group.  counters[0]
+ * counts the paths traced. */
+int orc_render_paths_adaptive(void *s, const orc_path_params *p, int group, float *mean, uint64_t *counters);
 
 /* ---- building blocks, exposed for known-answer tests ---- */
 void orc_numgen(uint64_t seed, uint64_t stream, int n, float *out);

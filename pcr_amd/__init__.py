@@ -19,7 +19,7 @@ HOST_LIB = os.path.join(BUILD, "libpcrhost.so")
 
 PCRHIP_OK, PCRHIP_EINVAL, PCRHIP_EDEVICE, PCRHIP_ENOMEM = 0, -1, -2, -3
 PCRHIP_ECANCELLED, PCRHIP_ETIMEOUT, PCRHIP_ESTATE = -4, -5, -6
-SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER, GUIDED = 1, 2, 4, 8, 16, 32
+SPECTRAL, MIS, RUSSIAN, STRATIFIED, JITTER, GUIDED, ADAPTIVE = 1, 2, 4, 8, 16, 32, 64
 SCHED_GRAPH, SCHED_PERSISTENT, SCHED_STAGED = 0, 1, 2
 ADMIT_SHORTEST, ADMIT_FIFO = 0, 1
 
@@ -67,7 +67,7 @@ class JobDesc(C.Structure):
                 ("shadow", C.c_int), ("hero", C.c_int), ("flags", C.c_uint32), ("seed", C.c_uint64),
                 ("x0", C.c_int), ("y0", C.c_int), ("cw", C.c_int), ("ch", C.c_int), ("priority", C.c_int),
                 ("guide_steps", C.c_int), ("guide_lr", C.c_float), ("guide_records", C.c_int),
-                ("guide_bounces", C.c_int)]
+                ("guide_bounces", C.c_int), ("adaptive_group", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -318,12 +318,13 @@ class DeviceScene:
 
 def job_desc(width, height, depth, spp, shadow, seed, spectral=True, hero=4, mis=False, russian=False,
              stratified=False, jitter=False, crop=None, priority=0, guided=False, guide_steps=0, guide_lr=0.0,
-             guide_records=0, guide_bounces=0):
+             guide_records=0, guide_bounces=0, adaptive_group=0):
     flags = (SPECTRAL if spectral else 0) | (MIS if mis else 0) | (RUSSIAN if russian else 0) | \
-            (STRATIFIED if stratified else 0) | (JITTER if jitter else 0) | (GUIDED if guided else 0)
+            (STRATIFIED if stratified else 0) | (JITTER if jitter else 0) | (GUIDED if guided else 0) | \
+            (ADAPTIVE if adaptive_group else 0)
     x0, y0, cw, ch = crop if crop else (0, 0, 0, 0)
     return JobDesc(width, height, depth, spp, shadow, hero, flags, seed, x0, y0, cw, ch, priority, guide_steps,
-                   guide_lr, guide_records, guide_bounces)
+                   guide_lr, guide_records, guide_bounces, adaptive_group)
 
 
 class Job:

@@ -32,6 +32,7 @@ enum : uint32_t {
     JF_STRATIFIED = 1u << 3,
     JF_JITTER = 1u << 4,
     JF_GUIDED = 1u << 5,
+    JF_ADAPTIVE = 1u << 6,
 };
 enum : uint32_t { MF_METALLIC = 1u, MF_TRANSPARENT = 2u, MF_TAB_ALBEDO = 4u, MF_TAB_EMISSIVE = 8u, MF_EMISSIVE = 16u };
 

@@ -25,7 +25,9 @@ struct DJob {
     int32_t spp_pass;        // samples per pixel in one pass
     int32_t s_begin;         // first sample index of the current pass
     float4 *radiance;        // [cw*ch*spp_pass] channel radiance of each path
-    double *accum;           // [cw*ch*3] running sums of XYZ (or RGB)
+    double *accum;           // [cw*ch*3] running sums of XYZ (or RGB); adaptive: sums of the open group
+    float4 *welford;         // adaptive: [cw*ch*2] (mean.xyz, m2.x), (m2.y, m2.z, points, done)
+    int32_t group;           // adaptive: samples per Welford point
 };
 
 // Structure-of-arrays ray queue (80 bytes per ray).

@@ -116,7 +116,8 @@ struct pcrhip_job {
     int adam_t = 0;
     std::vector<float> guide_loss;  // mean training loss per pass
     uint64_t bytes = 0;
-    std::vector<double> sums;
+    std::vector<double> sums;   // FP64 sums per pixel; adaptive: the Welford means
+    float4 *welford = nullptr;  // adaptive: device Welford state, after the sums in `accum`
     uint64_t counters[4] = {0, 0, 0, 0};
     Clock::time_point t_submit, t_admit, t_done;
     bool released = false;
@@ -219,6 +220,8 @@ int validate_job(const pcrhip_scene *s, const pcrhip_job_desc *d) {
     if ((uint64_t)d->width * d->height * (uint64_t)d->spp >= (1ull << 62))
         return fail(PCRHIP_EINVAL, "too many samples");
     if (s->S.n_lights < 1) return fail(PCRHIP_EINVAL, "scene has no light");
+    if ((d->flags & PCRHIP_ADAPTIVE) && (d->adaptive_group < 1 || d->spp % d->adaptive_group != 0))
+        return fail(PCRHIP_EINVAL, "adaptive sampling needs adaptive_group >= 1 dividing spp");
     if (d->guide_steps < 0 || d->guide_steps > kGMaxSteps) return fail(PCRHIP_EINVAL, "guide_steps must be in 0..128");
     if (!(d->guide_lr >= 0.f) || d->guide_records < 0 || d->guide_bounces < 0)
         return fail(PCRHIP_EINVAL, "bad guide_lr, guide_records or guide_bounces");
@@ -249,6 +252,8 @@ void fill_djob(const pcrhip_job *j, DJob &J) {
     J.s_begin = j->s_next;
     J.radiance = nullptr;  // set per pass: a window of the pass radiance buffer
     J.accum = j->accum;
+    J.welford = j->welford;
+    J.group = (d.flags & PCRHIP_ADAPTIVE) ? d.adaptive_group : 0;
 }
 
 constexpr uint64_t kPoolKeepBytes = 8ull << 30;  // freed accumulators kept for reuse
@@ -389,7 +394,10 @@ void admit(pcrhip_ctx *c) {
             continue;
         }
         uint64_t pix = (uint64_t)j->cw * j->ch;
-        uint64_t bytes = size_class(pix * 3 * sizeof(double));
+        // adaptive: a Welford record (2 float4) per pixel after the sums
+        const uint64_t wf_off = (pix * 3 * sizeof(double) + 15) & ~15ull;
+        const uint64_t raw = (j->d.flags & PCRHIP_ADAPTIVE) ? wf_off + pix * 2 * sizeof(float4) : pix * 3 * sizeof(double);
+        uint64_t bytes = size_class(raw);
         if (active > 0 && c->mem_in_use + bytes > c->cfg.memory_budget) break;  // wait for memory
         if (bytes > c->cfg.memory_budget) {
             c->pending.pop_front();
@@ -401,7 +409,8 @@ void admit(pcrhip_ctx *c) {
             if (!c->active[s]) { slot = s; break; }
         if (slot < 0) break;
         hipError_t e1 = pool_get(c, bytes, (void **)&j->accum);
-        hipError_t e2 = e1 == hipSuccess ? hipMemsetAsync(j->accum, 0, pix * 3 * sizeof(double), c->stream) : e1;
+        hipError_t e2 = e1 == hipSuccess ? hipMemsetAsync(j->accum, 0, raw, c->stream) : e1;
+        j->welford = (j->d.flags & PCRHIP_ADAPTIVE) && e1 == hipSuccess ? (float4 *)((char *)j->accum + wf_off) : nullptr;
         c->pending.pop_front();
         j->bytes = bytes;
         j->accum_bytes = e1 == hipSuccess ? bytes : 0;
@@ -795,7 +804,18 @@ void worker_main(pcrhip_ctx *c) {
             if (j->state == pcrhip_job::RUNNING && j->s_next >= j->d.spp) {
                 uint64_t n = (uint64_t)j->cw * j->ch * 3;
                 j->sums.resize(n);
-                hipError_t e = hipMemcpy(j->sums.data(), j->accum, n * sizeof(double), hipMemcpyDeviceToHost);
+                hipError_t e;
+                if (j->welford) {  // adaptive: the result is the Welford mean
+                    std::vector<float4> w(2 * (size_t)j->cw * j->ch);
+                    e = hipMemcpy(w.data(), j->welford, w.size() * sizeof(float4), hipMemcpyDeviceToHost);
+                    for (size_t p = 0; p < n / 3; p++) {
+                        j->sums[3 * p] = w[2 * p].x;
+                        j->sums[3 * p + 1] = w[2 * p].y;
+                        j->sums[3 * p + 2] = w[2 * p].z;
+                    }
+                } else {
+                    e = hipMemcpy(j->sums.data(), j->accum, n * sizeof(double), hipMemcpyDeviceToHost);
+                }
                 finish_job(c, j, e == hipSuccess ? PCRHIP_OK : PCRHIP_EDEVICE,
                            e == hipSuccess ? "" : hipGetErrorString(e));
             }
@@ -1333,7 +1353,8 @@ int pcrhip_job_result(pcrhip_job *j, float *out, size_t n, pcrhip_stats *st) {
     size_t need = (size_t)j->cw * j->ch * 3;
     if (out) {
         if (n < need) return fail(PCRHIP_EINVAL, "output buffer too small");
-        for (size_t i = 0; i < need; i++) out[i] = (float)(j->sums[i] / (double)j->d.spp);
+        const double div = j->welford ? 1.0 : (double)j->d.spp;  // adaptive sums hold the means
+        for (size_t i = 0; i < need; i++) out[i] = (float)(j->sums[i] / div);
     }
     return job_stats(j, st);
 }

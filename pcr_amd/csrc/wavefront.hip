@@ -129,6 +129,12 @@ __device__ __forceinline__ Bounce dielectric(V3 rd, V3 N, V3 hit, bool entering,
     return b;
 }
 
+// Adaptive sampling: the pixel of a pass path has converged (k_resolve set
+// its done word), so the path is not traced.
+__device__ __forceinline__ bool adaptive_done(const DJob &J, uint32_t path) {
+    const uint32_t pix = path / (uint32_t)J.spp_pass;
+    return __float_as_uint(ldg(J.welford + 2 * pix + 1).w) != 0u;
+}
 __device__ __forceinline__ void write_radiance(const DJob &J, const PathState &ps) {
     int nch = st_nch(ps.meta), ch0 = st_ch0(ps.meta);
     float *r = (float *)(J.radiance + ps.path);
@@ -567,9 +573,18 @@ void k_bounce(BounceArgs a) {
                 const uint32_t g_begin = __builtin_amdgcn_readfirstlane(s_segs[k0].begin);
                 const uint32_t g_count = __builtin_amdgcn_readfirstlane(s_segs[k0].count);
                 const DJob &J = a.jobs[s0];
-                if (FIRST) gen_path(J, s0, gpath, ps);
+                bool skip = false;
+                if (FIRST) {
+                    gen_path(J, s0, gpath, ps);
+                    skip = (J.flags & JF_ADAPTIVE) && adaptive_done(J, gpath);
+                }
                 const ForkSink fork{&a.out, a.count_out + k0, 4u * g_begin, 4u * g_count, a.overflow};
                 BounceCounters cnt{0u, 0u, 0u};
+                if (skip) {  // converged pixel: zero radiance, which k_resolve does not read
+                    write_radiance(J, ps);
+                    alive = false;
+                    continue;
+                }
                 if (LDS_SCENE) {
                     DScene Sl = J.S;
                     Sl.sph = s_geo + gl.sph;
@@ -643,7 +658,19 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
         double acc0 = ldg(dst), acc1 = ldg(dst + 1), acc2 = ldg(dst + 2);
         const bool spectral = (J.flags & JF_SPECTRAL) != 0;
         const int N = J.hero;
+        // adaptive: Welford state of the pixel (mean.xyz, m2.x), (m2.y, m2.z, points, done)
+        const bool adaptive = (J.flags & JF_ADAPTIVE) != 0;
+        float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t points = 0;
+        bool done = false;
+        if (adaptive) {
+            w0 = ldg(J.welford + 2 * pix);
+            w1 = ldg(J.welford + 2 * pix + 1);
+            points = __float_as_uint(w1.z);
+            done = __float_as_uint(w1.w) != 0u;
+        }
         for (int sl = 0; sl < J.spp_pass; sl++) {
+            if (done) break;  // samples after convergence are not taken
             uint32_t path = pix * (uint32_t)J.spp_pass + (uint32_t)sl;
             float4 r = ldg(J.radiance + path);
             float o0, o1, o2;
@@ -673,10 +700,36 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveArgs a) {
             acc0 += (double)o0;
             acc1 += (double)o1;
             acc2 += (double)o2;
+            if (adaptive && (J.s_begin + sl + 1) % J.group == 0) {
+                // the group (the reference's AA iteration) closes: its mean is
+                // one Welford point (float, as Renderer.cpp:266-294)
+                const float m[3] = {(float)(acc0 / (double)J.group), (float)(acc1 / (double)J.group),
+                                    (float)(acc2 / (double)J.group)};
+                float mean[3] = {w0.x, w0.y, w0.z}, m2[3] = {w0.w, w1.x, w1.y};
+                points++;
+                bool conv = points >= 4u;
+                _Pragma("unroll") for (int c = 0; c < 3; c++) {
+                    const float d = fsub(m[c], mean[c]);
+                    mean[c] = fadd(mean[c], div_rn(d, (float)points));
+                    m2[c] = fadd(m2[c], fmul(d, fsub(m[c], mean[c])));
+                    if (points >= 4u) {
+                        const float v = div_rn(m2[c], (float)(points - 1u));
+                        conv = conv && div_rn(v, fadd(fmul(mean[c], mean[c]), 0.01f)) < 0.05f;
+                    }
+                }
+                w0 = make_float4(mean[0], mean[1], mean[2], m2[0]);
+                w1 = make_float4(m2[1], m2[2], __uint_as_float(points), __uint_as_float(conv ? 1u : 0u));
+                done = conv;
+                acc0 = acc1 = acc2 = 0.0;
+            }
         }
         stg(dst, acc0);
         stg(dst + 1, acc1);
         stg(dst + 2, acc2);
+        if (adaptive) {
+            stg(J.welford + 2 * pix, w0);
+            stg(J.welford + 2 * pix + 1, w1);
+        }
     }
 }
 

@@ -54,7 +54,6 @@ void HipRenderer::render(const Scene &scene, std::chrono::steady_clock::time_poi
     lastOutputPath.clear();
     if (useDenoise) std::cerr << "warning: --denoise (bilateral) is not provided by the HIP backend; skipping.\n";
     if (useOIDN) std::cerr << "warning: --oidn is not provided by the HIP backend; skipping.\n";
-    if (useAdaptive) std::cerr << "warning: --adaptive is not provided by the HIP backend; rendering all samples.\n";
     if (!_ctx && pcrhip_create(device, &_ctx) != PCRHIP_OK) {
         std::cerr << "HIP backend: " << pcrhip_last_error() << std::endl;
         return;
@@ -80,6 +79,12 @@ void HipRenderer::render(const Scene &scene, std::chrono::steady_clock::time_poi
     if (useRussian) fl |= PCRHIP_RUSSIAN;
     if (useStratified) fl |= PCRHIP_STRATIFIED;
     if (aa > 1) fl |= PCRHIP_JITTER;
+    // adaptive: one Welford point per AA iteration of `samples` paths, as the
+    // reference GPU backend (MetalRenderer.mm:1368-1552)
+    if (useAdaptive) {
+        fl |= PCRHIP_ADAPTIVE;
+        jd.adaptive_group = std::max(1, _samples);
+    }
     if (useGuiding) fl |= PCRHIP_GUIDED;
     jd.flags = fl;
     // The reference GPU backends seed from the clock (OpenglRenderer.cpp:1995-1997).

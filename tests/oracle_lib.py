@@ -71,6 +71,7 @@ def lib():
         "orc_scene_intersect": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_float, _f, _i32, _i32, _i32, _f, _f]),
         "orc_render_branched": (C.c_int, [C.c_void_p, C.POINTER(BranchedParams), _f, _u8]),
         "orc_render_paths": (C.c_int, [C.c_void_p, C.POINTER(PathParams), _d, _u64]),
+        "orc_render_paths_adaptive": (C.c_int, [C.c_void_p, C.POINTER(PathParams), C.c_int, _f, _u64]),
         "orc_trace_one": (C.c_int, [C.c_void_p, C.POINTER(PathParams), C.c_int, C.c_int, C.c_int, _f, _f]),
         "orc_numgen": (None, [C.c_uint64, C.c_uint64, C.c_int, _f]),
         "orc_pcg_raw": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint32), _u64]),
@@ -208,6 +209,19 @@ class OracleScene:
         rc = lib().orc_render_paths(self.h, C.byref(p), acc.ctypes.data_as(_d), cnt.ctypes.data_as(_u64))
         assert rc == 0, rc
         return acc, cnt
+
+    def render_paths_adaptive(self, width, height, depth, spp, shadow, seed, group, jitter=False, hero=4,
+                              spectral=True, mis=False, russian=False, stratified=False, crop=None, threads=0):
+        """Welford means (ch, cw, 3) float32 and counters (paths traced first)."""
+        x0, y0, cw, ch = crop if crop else (0, 0, width, height)
+        p = PathParams(width, height, depth, spp, shadow, int(jitter), hero, int(spectral), int(mis),
+                       int(russian), int(stratified), seed, x0, y0, cw, ch, 0, spp, threads)
+        mean = np.zeros((ch, cw, 3), np.float32)
+        cnt = np.zeros(4, np.uint64)
+        rc = lib().orc_render_paths_adaptive(self.h, C.byref(p), group, mean.ctypes.data_as(_f),
+                                             cnt.ctypes.data_as(_u64))
+        assert rc == 0, rc
+        return mean, cnt
 
     def trace_one(self, params, px, py, s):
         rad = np.zeros(4, np.float32)
