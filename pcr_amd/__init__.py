@@ -119,6 +119,9 @@ HOST_API = {
     "pcrhost_scene_name": (C.c_char_p, [C.c_void_p]),
     "pcrhost_scene_bvh_depth": (C.c_int, [C.c_void_p]),
     "pcrhost_bvh_build": (C.c_int, [_f, C.c_int, _i32, _i32]),
+    "pcrhost_bvh_cache_dir": (None, [C.c_char_p]),
+    "pcrhost_bvh_cache_clear_memory": (None, []),
+    "pcrhost_bvh_cache_stats": (None, [C.POINTER(C.c_uint64)]),
     "pcrhost_fit_sigmoid": (None, [_f, _f]),
     "pcrhost_load_spd": (C.c_int, [C.c_char_p, _f]),
     "pcrhost_tonemap": (None, [_f, C.c_int, C.c_int, _u8]),
@@ -151,6 +154,23 @@ def hip():
     if _hip is None:
         _hip = _bind(HIP_LIB, HIP_API)
     return _hip
+
+
+def bvh_cache_dir(path):
+    """Directory of the on-disk BVH cache used by scene loads (None: memory
+    only).  The default is $PCR_BVH_CACHE."""
+    host().pcrhost_bvh_cache_dir(path.encode() if path else None)
+
+
+def bvh_cache_clear_memory():
+    host().pcrhost_bvh_cache_clear_memory()
+
+
+def bvh_cache_stats():
+    """(builds, memory hits, disk hits) of scene-load BVHs in this process."""
+    out = (C.c_uint64 * 3)()
+    host().pcrhost_bvh_cache_stats(out)
+    return tuple(int(x) for x in out)
 
 
 def host():

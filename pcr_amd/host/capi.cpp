@@ -39,7 +39,19 @@ const pcrhip_scene_desc *pcrhost_scene_desc(void *h) { return &((HostScene *)h)-
 const char *pcrhost_scene_name(void *h) { return ((HostScene *)h)->scene.name.c_str(); }
 int pcrhost_scene_bvh_depth(void *h) { return pcr::bvh_depth(((HostScene *)h)->scene.bvh); }
 
-// BVH over n triangles (9 floats each).  nodes: room for 2n-1 nodes of 9 int32;
+// Scene loads take their BVH from a content-keyed cache: in memory, and on
+// disk in `dir` (NULL or "" for memory only; $PCR_BVH_CACHE by default).
+void pcrhost_bvh_cache_dir(const char *dir) { pcr::bvh_cache_set_dir(dir); }
+void pcrhost_bvh_cache_clear_memory(void) { pcr::bvh_cache_clear_memory(); }
+// builds, memory hits, disk hits
+void pcrhost_bvh_cache_stats(uint64_t out[3]) {
+    const pcr::BvhCacheStats s = pcr::bvh_cache_stats();
+    out[0] = s.builds;
+    out[1] = s.memory_hits;
+    out[2] = s.disk_hits;
+}
+
+// BVH over n triangles (9 floats each), always built (no cache).  nodes: room for 2n-1 nodes of 9 int32;
 // perm: n.  Returns the node count.
 int pcrhost_bvh_build(const float *tris, int n, int32_t *nodes, int32_t *perm) {
     std::vector<pcr::Triangle> t((size_t)n);

@@ -284,7 +284,7 @@ Scene load_scene(const std::string &path, const std::vector<std::string> &spdDir
         }
     }
     if (s.lights.empty()) throw SceneError(path + ": at least one primitive must have \"light\": true; found none");
-    s.bvh = build_bvh(s.triangles);
+    s.bvh = build_bvh_cached(s.triangles);
     for (auto &m : s.materials) populate_spectra(m);
     return s;
 }

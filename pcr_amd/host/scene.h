@@ -92,6 +92,15 @@ bool load_spd_file(const std::string &path, std::array<float, 61> &out, std::str
 
 // bvh.cpp: bit-exact restatement of the reference builder (code/Bvh/Bvh.cpp:122-325)
 std::vector<BvhNode> build_bvh(std::vector<Triangle> &tris);
+// The same build through a content-keyed cache (in memory, and in the
+// directory of bvh_cache_set_dir or $PCR_BVH_CACHE when set).
+std::vector<BvhNode> build_bvh_cached(std::vector<Triangle> &tris);
+struct BvhCacheStats {
+    uint64_t builds, memory_hits, disk_hits;
+};
+void bvh_cache_set_dir(const char *dir);  // NULL or "" keeps builds in memory only
+void bvh_cache_clear_memory();
+BvhCacheStats bvh_cache_stats();
 int bvh_depth(const std::vector<BvhNode> &nodes);
 
 // loader.cpp
