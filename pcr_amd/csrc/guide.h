@@ -171,8 +171,8 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 struct GuideArgs {
     GuideNet net;
     QueueView q;
-    const uint32_t *count;  // live rays of the job's region (one job per guided pass)
-    uint32_t cap;           // rays the region holds
+    const uint32_t *count;  // the bounce's counters (one job per guided pass: segment 0 of each lineage)
+    uint32_t begin, paths;  // the job's queue region: [4 begin, 4 (begin + paths)), split by lineage
     GuideRecord *rec;
     uint32_t rec_cap;
     uint32_t keep;          // record a vertex if its 24-bit hash is below keep

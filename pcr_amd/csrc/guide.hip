@@ -286,18 +286,29 @@ struct GuideBlockLds {
 
 __global__ __launch_bounds__(64 * kGuideWaves) void k_guide(GuideArgs a) {
     __shared__ GuideBlockLds L;
-    const uint32_t n = min(*a.count, a.cap);
+    // Live rays of the job's lineages (segment 0 of each lineage's counters)
+    // and their exclusive prefix: item v of the launch is ray v - pre[l] of
+    // lineage l's sub-region.
+    uint32_t pre[kLineages + 1];
+    pre[0] = 0;
+    _Pragma("unroll") for (int l = 0; l < kLineages; l++) {
+        pre[l + 1] = pre[l] + min(a.count[l * kLinStride], 4u * lineage_count(a.paths, l));
+    }
+    const uint32_t n = pre[kLineages];
     if (blockIdx.x * (64u * kGuideWaves) >= n) return;  // uniform per block
     load_frags(a.net, L.W);
     const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6);
     float *io = L.io[wv];
     for (uint32_t base = (blockIdx.x * kGuideWaves + wv) * 64u; base < n;
          base += gridDim.x * (64u * kGuideWaves)) {
-        const uint32_t i = base + (uint32_t)lane;
+        const uint32_t v = base + (uint32_t)lane;
+        uint32_t i = 0;  // queue index of item v
+        _Pragma("unroll") for (int l = 0; l < kLineages; l++)
+            if (v >= pre[l]) i = 4u * (a.begin + lineage_base(a.paths, l)) + (v - pre[l]);
         float4 od1 = make_float4(0.f, 0.f, 1.f, 0.f), od0 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < n) od1 = ldg(a.q.od1 + i);
+        if (v < n) od1 = ldg(a.q.od1 + i);
         uint32_t meta = __float_as_uint(od1.w);
-        const bool pend = i < n && (meta & ST_GUIDE) != 0;
+        const bool pend = v < n && (meta & ST_GUIDE) != 0;
         if (!__ballot(pend)) continue;  // uniform: the whole wave skips
         float4 tp = make_float4(0.f, 0.f, 0.f, 0.f);
         uint4 rg = make_uint4(0u, 0u, 0u, 0u);

@@ -53,9 +53,44 @@ struct ResolveSeg {
 
 constexpr int kMaxSegs = 64;
 
-// Sizes of the current pass, read by the kernels from device memory so that
-// the launch arguments stay the same from pass to pass (a captured pass
-// graph is replayed unchanged).
+// Ray lineages.  The paths of every job in a pass are dealt to kLineages
+// lineages in granules of kLinGranule consecutive paths (granule g goes to
+// lineage g % kLineages, so each lineage samples the whole image evenly); the
+// rays descending from a lineage's paths (continuations and dispersion forks)
+// stay in its sub-region of the job's queue region, which has room for 4 rays
+// per path of the lineage (a path forks once, into at most 4).  Each lineage
+// has its own append counter per job and its own work counter, so the per-ray
+// atomics of a bounce are spread over kLineages addresses instead of one (a
+// single address serialises them).  A bounce block serves lineage
+// blockIdx % kLineages first -- the hardware deals consecutive blocks
+// round-robin over the 8 XCDs, so a lineage's queue traffic and counters stay
+// in one XCD's L2 -- and then helps the other lineages until all are drained.
+constexpr int kLineages = 8;
+constexpr uint32_t kLinGranule = 64;
+// Counters of one bounce: lineage j's append counters (one per segment) at
+// j * kLinStride, its work counter at j * kLinStride + kMaxSegs; every
+// lineage's counters start on a fresh 128-byte line.
+constexpr int kLinStride = kMaxSegs + 32;
+constexpr int kCtrStride = kLineages * kLinStride;
+static_assert(kLinStride % 32 == 0, "lineage counters are line aligned");
+// Paths of lineage j in a segment of `count` paths.
+__host__ __device__ inline uint32_t lineage_count(uint32_t count, int j) {
+    const uint32_t full = count / kLinGranule, rem = count % kLinGranule;
+    uint32_t c = full > (uint32_t)j ? ((full - 1u - (uint32_t)j) / kLineages + 1u) * kLinGranule : 0u;
+    if (rem != 0u && full % kLineages == (uint32_t)j) c += rem;
+    return c;
+}
+// Paths of the lineages before j: lineage j's queue sub-region starts at 4 times this.
+__host__ __device__ inline uint32_t lineage_base(uint32_t count, int j) {
+    uint32_t b = 0;
+    for (int i = 0; i < j; i++) b += lineage_count(count, i);
+    return b;
+}
+// Segment-local path of the r-th path of lineage j.
+__host__ __device__ inline uint32_t lineage_path(int j, uint32_t r) {
+    return ((r / kLinGranule) * kLineages + (uint32_t)j) * kLinGranule + r % kLinGranule;
+}
+
 struct PassInfo {
     uint32_t n_gen;    // generating bounce: number of work items (paths)
     int32_t n_segs;    // GenSeg entries
@@ -97,9 +132,9 @@ struct BounceArgs {
     const GenSeg *segs;
     const PassInfo *pass;
     QueueView in, out;
-    const uint32_t *count_in;     // [kMaxSegs] live rays per segment in `in`
-    uint32_t *count_out;          // [kMaxSegs] rays appended per segment to `out`
-    uint32_t *work;               // work-stealing counter of this launch
+    const uint32_t *count_in;     // previous bounce's counters: live rays per (lineage, segment) in `in`
+    uint32_t *count_out;          // this bounce's counters: rays appended per (lineage, segment) to `out`,
+                                  // and the lineages' work-stealing counters
     uint32_t *overflow;           // set if an append would exceed its region
     unsigned long long *counters; // paths, extension rays, shadow rays, emissive hits
 };

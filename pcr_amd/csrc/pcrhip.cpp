@@ -75,7 +75,6 @@ float ibits(int32_t v) {
 
 constexpr int kSlots = 64;
 constexpr int kMaxDepthBounces = 255;
-constexpr int kCtrStride = kMaxSegs + 1;  // counters of one bounce
 
 }  // namespace
 
@@ -465,7 +464,6 @@ BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
     a.out = (b & 1) ? c->qb : c->qa;
     a.count_in = b == 0 ? nullptr : c->ctrs + kCtrStride * (b - 1);
     a.count_out = c->ctrs + kCtrStride * b;
-    a.work = c->ctrs + kCtrStride * b + kMaxSegs;
     a.overflow = c->ctrs + kCtrStride * sh.nb;
     a.counters = c->slot_counters;
     return a;
@@ -486,12 +484,15 @@ int enqueue_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hipEvent_
         if (staged) {
             uint64_t live = n_gen;
             if (b > 0) {
-                uint32_t cnt[kMaxSegs];
-                HIPCHK(hipMemcpyAsync(cnt, a.count_in, segs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                      c->stream));
+                uint32_t cnt[kCtrStride];
+                HIPCHK(hipMemcpyAsync(cnt, a.count_in, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipStreamSynchronize(c->stream));
                 live = 0;
-                for (size_t k = 0; k < segs.size(); k++) live += std::min<uint64_t>(cnt[k], 4ull * segs[k].count);
+                for (size_t k = 0; k < segs.size(); k++)
+                    for (int l = 0; l < kLineages; l++)
+                        live += std::min<uint64_t>(
+                            cnt[l * kLinStride + k],
+                            4ull * lineage_count(segs[k].count, l));
             }
             if (live == 0) break;  // every path has ended
             grid = (int)std::min<uint64_t>((live + eb - 1) / eb, 1u << 30);
@@ -534,7 +535,7 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
         // rays leaving launch b carry bounce index b: only the guided ones pend
         const int gb = j->d.guide_bounces > 0 ? j->d.guide_bounces : kGDefaultBounces;
         if (b >= gb) continue;
-        GuideArgs ga{g, a.out, a.count_out, 4u * segs[0].count, c->grec, rec_cap, keep, j->adam_t > 0};
+        GuideArgs ga{g, a.out, a.count_out, segs[0].begin, segs[0].count, c->grec, rec_cap, keep, j->adam_t > 0};
         HIPCHK(pcr_launch_guide(&ga, c->n_cu, c->stream));
     }
     ResolveArgs ra{c->jobs_dev, c->rsegs_dev, c->pass_dev};
@@ -724,7 +725,8 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
             c->prof.bounce_ms += ms;
             c->prof.bounce_launches++;
             uint64_t written = 0;
-            for (size_t k = 0; k < segs.size(); k++) written += ctr[kCtrStride * b + k];
+            for (size_t k = 0; k < segs.size(); k++)
+                for (int l = 0; l < kLineages; l++) written += ctr[kCtrStride * b + l * kLinStride + k];
             c->prof.queue_bytes += (b == 0 ? 0 : rays_in * 80) + written * 80;
             rays_in = written;
         }

@@ -414,17 +414,39 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Work item -> segment: the largest k with pref[k] <= item (pref is the
-// exclusive prefix of the segments' item counts, in LDS; empty segments are
-// skipped because their successor has the same prefix).
-__device__ __forceinline__ int seg_of(const uint32_t *pref, int nseg, uint32_t item) {
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (pref[mid] <= item) lo = mid;
-        else hi = mid - 1;
+// Work item -> segment: the largest k < nseg with pref(k) <= item, where
+// lane k of pref_l holds pref(k), the exclusive prefix of the segments' item
+// counts (empty segments are skipped because their successor has the same
+// prefix).  A fixed-trip binary search over lanes; every lane must be active.
+__device__ __forceinline__ int seg_of(uint32_t pref_l, int nseg, uint32_t item) {
+    int pos = 0;
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int cand = pos + w;
+        const uint32_t pv = (uint32_t)__shfl((int)pref_l, cand < 64 ? cand : 63, 64);
+        if (cand < nseg && pv <= item) pos = cand;
     }
-    return lo;
+    return pos;
+}
+// Lane k of the result: the exclusive prefix, over the segments, of segment
+// k's input items in lineage `lin` (*total: all of them).  Every lane must be
+// active.
+template <bool FIRST>
+__device__ __forceinline__ uint32_t lineage_prefix(const BounceArgs &a, const GenSeg *segs, int n_segs, int lin,
+                                                   uint32_t *total) {
+    const int k = (int)__lane_id();
+    uint32_t v = 0;
+    if (k < n_segs) {
+        const uint32_t np = lineage_count(segs[k].count, lin);
+        // an overflowing append still counts: never read past the sub-region
+        v = FIRST ? np : min(a.count_in[lin * kLinStride + k], 4u * np);
+    }
+    uint32_t incl = v;
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        uint32_t t = __shfl_up(incl, dd, 64);
+        if (k >= dd) incl += t;
+    }
+    *total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    return incl - v;
 }
 
 // Per-lane counts for the job slot the wave is currently working on.  When
@@ -454,11 +476,16 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 
 // Queue layout: every job of the pass (segment k of the GenSeg table) owns the
 // region [4*begin_k, 4*(begin_k + count_k)) of both ping-pong queues -- room
-// for every path of the job to fork into its 4 hero channels -- and a live-ray
-// counter per bounce.  A launch walks the concatenation of the segments' live
-// rays in 128-ray chunks per wave, so a wave almost always holds rays of one job: the
-// job's descriptor and scene are read with scalar loads and the shading code
-// runs once per wave.
+// for every path of the job to fork into its 4 hero channels -- split into
+// kLineages sub-regions, one per lineage (a contiguous range of the job's
+// paths, 4 rays per path), each with a live-ray counter per bounce.  Within a
+// lineage a launch walks the concatenation of the segments' live rays in
+// 128-ray chunks per wave, so a wave almost always holds rays of one job: the
+// job's descriptor and scene are read with scalar loads (the job table is
+// passed as a restrict pointer the kernel never writes) and the shading code
+// runs once per wave.  Lineages keep the appends off a single counter: one
+// address took every wave's atomic, and its serialisation set the launch time
+// of small scenes.
 // Occupancy: 3 waves per SIMD (<= 168 VGPRs, a few dozen bytes of spills)
 // measured 20% faster than the unconstrained 2 waves at 185-195 VGPRs, and 4
 // waves (128 VGPRs, ~200 bytes of spills) slower.  Up to 24 stack entries,
@@ -469,10 +496,9 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // traversal, materials, lights and spectral tables are read there.
 template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 24 ? 3 : 1, STACK <= 24 ? 3 : 10)))
-void k_bounce(BounceArgs a) {
+void k_bounce(BounceArgs a, const DJob *__restrict__ jobs) {
     __shared__ int32_t s_idx[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
-    __shared__ uint32_t s_pref[kMaxSegs + 1];
     __shared__ GenSeg s_segs[kMaxSegs];
     __shared__ float4 s_geo[LDS_SCENE ? geo_cap(BLOCK) : 1];
     using GA = typename std::conditional<LDS_SCENE, GeoLds, GeoGlobal>::type;
@@ -481,7 +507,7 @@ void k_bounce(BounceArgs a) {
     const uint32_t lane = lane_id();
     GeoLayout gl{};
     if (LDS_SCENE) {  // the host checked geo_layout(S0).total <= geo_cap(BLOCK)
-        const DScene &S0 = a.jobs[a.segs[0].slot].S;
+        const DScene &S0 = jobs[a.segs[0].slot].S;
         gl = geo_layout(S0);
         auto copy4 = [&](int off, const void *src, int n4) {
             for (int i = threadIdx.x; i < n4; i += BLOCK) s_geo[off + i] = ldg((const float4 *)src + i);
@@ -498,57 +524,54 @@ void k_bounce(BounceArgs a) {
         copy4(gl.ltri, S0.ltri, 4 * S0.n_ltri);
         for (int i = threadIdx.x; i < S0.n_tabs; i += BLOCK) ((float *)(s_geo + gl.tabs))[i] = ldg(S0.tabs + i);
     }
-    // The segment table and the exclusive prefix of the segments' input
-    // counts, staged in LDS (one wave scans <= 64 segments): every ray's queue
-    // address depends on them, so they must not cost a memory round trip.
-    if (threadIdx.x < 64) {
-        const int k = (int)threadIdx.x;
-        uint32_t v = 0;
-        if (k < n_segs) {
-            const GenSeg g = a.segs[k];
-            s_segs[k] = g;
-            // an overflowing append still counts: never read past the region
-            v = FIRST ? g.count : min(a.count_in[k], 4u * g.count);
-        }
-        uint32_t incl = v;
-        for (int dd = 1; dd < 64; dd <<= 1) {
-            uint32_t t = __shfl_up(incl, dd, 64);
-            if (k >= dd) incl += t;
-        }
-        if (k < n_segs) s_pref[k] = incl - v;
-        if (k == 63) s_pref[kMaxSegs] = incl;
-    }
+    // The segment table, staged in LDS: every ray's queue address depends on
+    // it, so it must not cost a memory round trip.  Each wave keeps the
+    // segments' input prefix of its current lineage in a register (lane k:
+    // segment k), rescanned when it moves to another lineage.
+    for (int k = threadIdx.x; k < n_segs; k += BLOCK) s_segs[k] = a.segs[k];
     __syncthreads();
-    const uint32_t n_in = s_pref[kMaxSegs];
     LaneCounters lc;
     int wslot = -1;  // job slot of the counts in lc (wave-uniform)
-    // Each wave claims its own chunks of 128 rays and takes them 64 at a time:
-    // no block barrier per chunk, so a wave with long paths (forks, NEE) does
-    // not hold up the others, and one atomic serves two rounds.
+    // Each wave claims its own chunks of 128 rays from its lineage's work
+    // counter and takes them 64 at a time: no block barrier per chunk, so a
+    // wave with long paths (forks, NEE) does not hold up the others, and one
+    // atomic serves two rounds.  A drained lineage passes the wave on to the
+    // next one; the wave exits when it has found all of them drained.
+    int lin = (int)(blockIdx.x % kLineages), drained = 0;  // wave-uniform
+    uint32_t n_in;                                         // wave-uniform
+    uint32_t pref_l = lineage_prefix<FIRST>(a, s_segs, n_segs, lin, &n_in);
     uint32_t chunk = 0, rounds = 0;  // wave-uniform
     for (;;) {
         if (rounds == 0) {
             uint32_t c = 0;
-            if (lane == 0) c = atomicAdd(a.work, 128u);
+            if (lane == 0) c = atomicAdd(a.count_out + lin * kLinStride + kMaxSegs, 128u);
             chunk = __builtin_amdgcn_readfirstlane(c);
             rounds = 2;
         }
         const uint32_t base = chunk;
         chunk += 64u;
         rounds--;
-        if (base >= n_in) break;
+        if (base >= n_in) {
+            if (++drained == kLineages) break;
+            lin = lin + 1 == kLineages ? 0 : lin + 1;
+            pref_l = lineage_prefix<FIRST>(a, s_segs, n_segs, lin, &n_in);
+            rounds = 0;
+            continue;
+        }
         const uint32_t i = base + lane;
         PathState ps;
         bool ok = false;
-        int seg = -1;
         uint32_t gpath = 0;
+        // segment of the lane's item, with every lane taking part in the shuffles
+        const uint32_t ic = i < n_in ? i : n_in - 1u;
+        int seg = n_segs == 1 ? 0 : seg_of(pref_l, n_segs, ic);
+        const uint32_t r = ic - (uint32_t)__shfl((int)pref_l, seg, 64);
+        if (i >= n_in) seg = -1;
         if (i < n_in) {
-            seg = n_segs == 1 ? 0 : seg_of(s_pref, n_segs, i);
-            const uint32_t r = i - s_pref[seg];
             if (FIRST) {
-                gpath = s_segs[seg].path0 + r;
+                gpath = s_segs[seg].path0 + lineage_path(lin, r);
             } else {
-                st_load(a.in, 4u * s_segs[seg].begin + r, ps);
+                st_load(a.in, 4u * (s_segs[seg].begin + lineage_base(s_segs[seg].count, lin)) + r, ps);
             }
             ok = true;
         }
@@ -572,13 +595,15 @@ void k_bounce(BounceArgs a) {
                 pending = false;
                 const uint32_t g_begin = __builtin_amdgcn_readfirstlane(s_segs[k0].begin);
                 const uint32_t g_count = __builtin_amdgcn_readfirstlane(s_segs[k0].count);
-                const DJob &J = a.jobs[s0];
+                const DJob &J = jobs[s0];
                 bool skip = false;
                 if (FIRST) {
                     gen_path(J, s0, gpath, ps);
                     skip = (J.flags & JF_ADAPTIVE) && adaptive_done(J, gpath);
                 }
-                const ForkSink fork{&a.out, a.count_out + k0, 4u * g_begin, 4u * g_count, a.overflow};
+                const uint32_t l_lo = lineage_base(g_count, lin), l_n = lineage_count(g_count, lin);
+                const ForkSink fork{&a.out, a.count_out + lin * kLinStride + k0, 4u * (g_begin + l_lo), 4u * l_n,
+                                    a.overflow};
                 BounceCounters cnt{0u, 0u, 0u};
                 if (skip) {  // converged pixel: zero radiance, which k_resolve does not read
                     write_radiance(J, ps);
@@ -618,14 +643,16 @@ void k_bounce(BounceArgs a) {
             const uint32_t off = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             const uint32_t total = (uint32_t)__popcll(m);
             uint32_t wbase = 0;
-            if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) wbase = atomicAdd(a.count_out + k0, total);
+            if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                wbase = atomicAdd(a.count_out + lin * kLinStride + k0, total);
             wbase = __shfl(wbase, __ffsll((unsigned long long)m) - 1, 64);
             if (mine) {
                 app = false;
                 const uint32_t g_begin = __builtin_amdgcn_readfirstlane(s_segs[k0].begin);
                 const uint32_t g_count = __builtin_amdgcn_readfirstlane(s_segs[k0].count);
+                const uint32_t l_lo = lineage_base(g_count, lin), l_n = lineage_count(g_count, lin);
                 const uint32_t dst = wbase + off;
-                if (dst < 4u * g_count) st_store(a.out, 4u * g_begin + dst, ps);
+                if (dst < 4u * l_n) st_store(a.out, 4u * (g_begin + l_lo) + dst, ps);
                 else atomicOr(a.overflow, 1u);
             }
         }
@@ -776,7 +803,7 @@ __global__ void k_reset(uint32_t *p, int n) {
 // ------------------------------------------------------------- launchers --
 template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
 static hipError_t launch_bounce_t(const BounceArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_bounce<FIRST, STACK, BLOCK, LDS_SCENE>), dim3(grid), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_bounce<FIRST, STACK, BLOCK, LDS_SCENE>), dim3(grid), dim3(BLOCK), 0, s, a, a.jobs);
     return hipGetLastError();
 }
 template <int STACK, bool LDS_SCENE>

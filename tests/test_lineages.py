@@ -1,0 +1,21 @@
+"""Lineage partition of a pass segment's paths (pcr_amd/csrc/pcr_kernels.h).
+
+The bounce kernels deal a job's paths to ray lineages in granules and give
+each lineage its own queue sub-region and counters.  The partition must cover
+every path exactly once and the sub-regions must tile the job's region; the
+check runs the header's own host/device functions on the CPU.
+"""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_lineages_partition_paths(tmp_path):
+    exe = str(tmp_path / "lineage_check")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O1", "-std=c++20", "--cuda-host-only", "-x", "hip",
+                    "-I" + os.path.join(ROOT, "pcr_amd", "csrc"), "-I" + os.path.join(ROOT, "include"),
+                    "-o", exe, os.path.join(ROOT, "tests", "csrc", "lineage_check.cpp")],
+                   check=True, capture_output=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
