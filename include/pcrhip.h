@@ -203,7 +203,9 @@ typedef struct {
     int priority;        /* higher is admitted first */
     /* PCRHIP_GUIDED training per pass (0 selects the default): Adam steps
      * (<= 128, default 128), learning rate (default 5e-3), training samples
-     * recorded (default 262144).  guide_bounces: the vertices of bounce index
+     * recorded (default 262144; recorded in 8 partitions of guide_records / 8,
+     * one per group of the guiding kernel's blocks).  guide_bounces: the
+     * vertices of bounce index
      * below it draw from the learned mixture, later ones sample the cosine
      * density alone (default 3). */
     int guide_steps;

@@ -131,7 +131,8 @@ struct GuideNet {
     float *part;       // [kGTrainBlocks][kGParams] per-block gradient sums of a step
     float *m, *v;      // Adam moments
     float *wsum;       // [2 * kGMaxSteps]: per training step, weight total and loss
-    uint32_t *n_rec;   // records written this pass
+    uint32_t *n_rec;   // records written this pass (after k_guide_pack: contiguous at the start of the buffer)
+    uint32_t *n_part;  // [kLineages * 32] records written per staging partition, one 128-byte line each
     int32_t *step;     // Adam steps taken
     float lo[3], inv[3];  // position normalisation: (p - lo) * inv in [0, 1]
 };
@@ -173,8 +174,8 @@ struct GuideArgs {
     QueueView q;
     const uint32_t *count;  // the bounce's counters (one job per guided pass: segment 0 of each lineage)
     uint32_t begin, paths;  // the job's queue region: [4 begin, 4 (begin + paths)), split by lineage
-    GuideRecord *rec;
-    uint32_t rec_cap;
+    GuideRecord *rec;       // 2 rec_cap records: k_guide writes to the staging half [rec_cap, 2 rec_cap),
+    uint32_t rec_cap;       // partition blockIdx % kLineages holding rec_cap / kLineages of them
     uint32_t keep;          // record a vertex if its 24-bit hash is below keep
     int mix;                // 0: the network is untrained, sample the cosine density alone
 };
@@ -182,6 +183,9 @@ struct GuideArgs {
 
 extern "C" {
 hipError_t pcr_launch_guide(const pcr::GuideArgs *a, int grid, hipStream_t s);
+// Moves the records of the kLineages staging partitions (rec + rec_cap) to the
+// start of rec, contiguously, and sets *g->n_rec.
+hipError_t pcr_launch_guide_pack(const pcr::GuideNet *g, pcr::GuideRecord *rec, uint32_t rec_cap, hipStream_t s);
 hipError_t pcr_launch_guide_prep(const pcr::GuideNet *g, pcr::GuideRecord *rec, uint32_t rec_cap,
                                  const float4 *radiance, int S, hipStream_t s);
 hipError_t pcr_launch_guide_train(const pcr::GuideNet *g, const pcr::GuideRecord *rec, uint32_t rec_cap, int step,

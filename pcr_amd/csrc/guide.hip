@@ -377,16 +377,20 @@ __global__ __launch_bounds__(64 * kGuideWaves) void k_guide(GuideArgs a) {
             stg(a.q.tp + i, tp);
             a.q.rng[i] = rg;
         }
-        // training records: one counter update per wave
+        // training records: one counter update per wave, on the block's
+        // staging partition (one counter for the whole launch serialised the
+        // waves' atomics)
         const uint64_t rb = __ballot(rec);
         if (rb) {
+            const uint32_t part = blockIdx.x % kLineages, pcap = a.rec_cap / kLineages;
             uint32_t first = 0;
-            if (lane == __ffsll((unsigned long long)rb) - 1) first = atomicAdd(a.net.n_rec, (uint32_t)__popcll(rb));
+            if (lane == __ffsll((unsigned long long)rb) - 1)
+                first = atomicAdd(a.net.n_part + 32 * part, (uint32_t)__popcll(rb));
             first = __shfl(first, __ffsll((unsigned long long)rb) - 1, 64);
             const uint32_t slot = first + (uint32_t)__popcll(rb & ((1ull << lane) - 1ull));
-            if (rec && slot < a.rec_cap) {
+            if (rec && slot < pcap) {
                 const float4 rd = ldg(a.q.rad + i);
-                GuideRecord &R = a.rec[slot];
+                GuideRecord &R = a.rec[a.rec_cap + part * pcap + slot];
                 R.o[0] = od0.x; R.o[1] = od0.y; R.o[2] = od0.z;
                 R.n[0] = N.x; R.n[1] = N.y; R.n[2] = N.z;
                 R.w[0] = w.x; R.w[1] = w.y; R.w[2] = w.z;
@@ -398,6 +402,27 @@ __global__ __launch_bounds__(64 * kGuideWaves) void k_guide(GuideArgs a) {
                 R.weight = 0.f;
             }
         }
+    }
+}
+
+// The staging partitions' records, concatenated in partition order at the
+// start of the buffer (the training kernels read records [0, n_rec)).
+__global__ void k_guide_pack(GuideNet g, GuideRecord *rec, uint32_t rec_cap) {
+    const uint32_t pcap = rec_cap / kLineages;
+    uint32_t pre[kLineages + 1];
+    pre[0] = 0;
+    _Pragma("unroll") for (int l = 0; l < kLineages; l++) pre[l + 1] = pre[l] + min(g.n_part[32 * l], pcap);
+    const uint32_t n = pre[kLineages];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *g.n_rec = n;
+    const float4 *stage = (const float4 *)(rec + rec_cap);
+    float4 *dst = (float4 *)rec;
+    constexpr uint32_t kW = sizeof(GuideRecord) / sizeof(float4);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n * kW; t += gridDim.x * blockDim.x) {
+        const uint32_t r = t / kW, w = t - r * kW;
+        uint32_t src = 0;
+        _Pragma("unroll") for (int l = 0; l < kLineages; l++)
+            if (r >= pre[l]) src = (uint32_t)l * pcap + (r - pre[l]);
+        dst[t] = ldg(stage + (size_t)src * kW + w);
     }
 }
 
@@ -762,6 +787,10 @@ __global__ __launch_bounds__(64 * kGuideWaves) void k_guide_forward(GuideNet g, 
 using namespace pcr;
 extern "C" hipError_t pcr_launch_guide(const GuideArgs *a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_guide, dim3(grid), dim3(64 * kGuideWaves), 0, s, *a);
+    return hipGetLastError();
+}
+extern "C" hipError_t pcr_launch_guide_pack(const GuideNet *g, GuideRecord *rec, uint32_t rec_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_pack, dim3(512), dim3(256), 0, s, *g, rec, rec_cap);
     return hipGetLastError();
 }
 extern "C" hipError_t pcr_launch_guide_prep(const GuideNet *g, GuideRecord *rec, uint32_t rec_cap,

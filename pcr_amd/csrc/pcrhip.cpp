@@ -332,7 +332,7 @@ std::vector<float> guide_init(uint64_t seed) {
 // Every region starts 16-byte aligned (the fragment loads are 16 bytes).
 size_t guide_bytes() {
     return (4 + (size_t)kGTrainBlocks) * kGParams * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 +
-           (size_t)(kGFragElems + kGBackElems) * sizeof(uint16_t);
+           kLineages * 32 * sizeof(uint32_t) + (size_t)(kGFragElems + kGBackElems) * sizeof(uint16_t);
 }
 void guide_layout(GuideNet &g, void *mem) {
     const size_t P = kGParams;
@@ -343,11 +343,13 @@ void guide_layout(GuideNet &g, void *mem) {
     g.wsum = g.v + P;
     g.n_rec = (uint32_t *)(g.wsum + 2 * kGMaxSteps);
     g.step = (int32_t *)(g.n_rec + 1);
-    g.wf = (uint16_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+    g.n_part = (uint32_t *)((char *)g.wsum + 2 * kGMaxSteps * sizeof(float) + 64);
+    g.wf = (uint16_t *)((char *)g.n_part + kLineages * 32 * sizeof(uint32_t));
     g.wk = g.wf + kGFragElems;
     g.part = (float *)(g.wk + kGBackElems);
 }
-static_assert((4 * kGParams * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64) % 16 == 0 &&
+static_assert((4 * kGParams * sizeof(float) + 2 * kGMaxSteps * sizeof(float) + 64 +
+               kLineages * 32 * sizeof(uint32_t)) % 16 == 0 &&
                   (kGFragElems * sizeof(uint16_t)) % 16 == 0 && (kGBackElems * sizeof(uint16_t)) % 16 == 0,
               "guide regions stay 16-byte aligned");
 
@@ -524,6 +526,7 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
     HIPCHK(pcr_launch_reset((uint32_t *)c->slot_counters, kSlots * 4 * 2, c->stream));
     HIPCHK(pcr_launch_reset((uint32_t *)g.wsum, 2 * kGMaxSteps, c->stream));
     HIPCHK(pcr_launch_reset(g.n_rec, 1, c->stream));
+    HIPCHK(pcr_launch_reset(g.n_part, kLineages * 32, c->stream));
     // record about rec_cap of the pass's guided vertices (2 per path assumed)
     const double frac = std::min(1.0, (double)rec_cap / (2.0 * std::max<uint32_t>(n_gen, 1)));
     const uint32_t keep = (uint32_t)std::min(16777216.0, std::ceil(frac * 16777216.0));
@@ -542,6 +545,7 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
     if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb], c->stream));
     HIPCHK(pcr_launch_resolve(&ra, sh.rgrid, c->stream));
     if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb + 1], c->stream));
+    HIPCHK(pcr_launch_guide_pack(&g, c->grec, rec_cap, c->stream));
     HIPCHK(pcr_launch_guide_prep(&g, c->grec, rec_cap, c->pass_rad + segs[0].begin, S, c->stream));
     GuideTrain h{j->d.guide_lr > 0.f ? j->d.guide_lr : 5e-3f, 0.9f, 0.99f, 1e-8f, 0.f, 0.f};
     for (int q = 0; q < S; q++) {
@@ -661,7 +665,8 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
         (void)hipFree(c->grec);
         c->grec = nullptr;
         c->grec_cap = 0;
-        HIPCHK(hipMalloc(&c->grec, (size_t)rec_cap * sizeof(GuideRecord)));
+        // the records, then the staging partitions k_guide writes them to
+        HIPCHK(hipMalloc(&c->grec, 2 * (size_t)rec_cap * sizeof(GuideRecord)));
         c->grec_cap = rec_cap;
     }
     *c->pass_host = PassInfo{(uint32_t)n_gen, (int32_t)segs.size(), (uint32_t)n_pix, (int32_t)rsegs.size()};
