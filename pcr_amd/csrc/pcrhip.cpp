@@ -144,6 +144,8 @@ struct pcrhip_ctx {
     uint64_t pool_bytes = 0;
     DJob *jobs_host = nullptr;  // pinned staging of the job table
     PassInfo *pass_host = nullptr;  // pinned staging of the pass sizes
+    double *sums_host = nullptr;    // pinned staging of a finished job's FP64 sums (worker thread only)
+    size_t sums_host_cap = 0;
     // captured pass graphs, keyed by launch shape; dropped when the queues move
     struct PassGraph {
         int nb, stack, block, grid;
@@ -810,9 +812,9 @@ void worker_main(pcrhip_ctx *c) {
         for (auto *j : run) {
             if (j->state == pcrhip_job::RUNNING && j->s_next >= j->d.spp) {
                 uint64_t n = (uint64_t)j->cw * j->ch * 3;
-                j->sums.resize(n);
                 hipError_t e;
                 if (j->welford) {  // adaptive: the result is the Welford mean
+                    j->sums.resize(n);
                     std::vector<float4> w(2 * (size_t)j->cw * j->ch);
                     e = hipMemcpy(w.data(), j->welford, w.size() * sizeof(float4), hipMemcpyDeviceToHost);
                     for (size_t p = 0; p < n / 3; p++) {
@@ -821,7 +823,21 @@ void worker_main(pcrhip_ctx *c) {
                         j->sums[3 * p + 2] = w[2 * p].z;
                     }
                 } else {
-                    e = hipMemcpy(j->sums.data(), j->accum, n * sizeof(double), hipMemcpyDeviceToHost);
+                    // through pinned staging: a copy into pageable memory runs at a fraction of the link
+                    // rate, and the GPU waits for it between frames
+                    e = hipSuccess;
+                    if (n > c->sums_host_cap) {
+                        (void)hipHostFree(c->sums_host);
+                        c->sums_host = nullptr;
+                        c->sums_host_cap = 0;
+                        e = hipHostMalloc(&c->sums_host, n * sizeof(double), 0);
+                        if (e == hipSuccess) c->sums_host_cap = n;
+                    }
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(c->sums_host, j->accum, n * sizeof(double), hipMemcpyDeviceToHost,
+                                           c->stream);
+                    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                    if (e == hipSuccess) j->sums.assign(c->sums_host, c->sums_host + n);
                 }
                 finish_job(c, j, e == hipSuccess ? PCRHIP_OK : PCRHIP_EDEVICE,
                            e == hipSuccess ? "" : hipGetErrorString(e));
@@ -1165,6 +1181,7 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     (void)hipFree(c->ctrs);
     (void)hipFree(c->slot_counters);
     (void)hipHostFree(c->jobs_host);
+    (void)hipHostFree(c->sums_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return PCRHIP_OK;
@@ -1360,8 +1377,15 @@ int pcrhip_job_result(pcrhip_job *j, float *out, size_t n, pcrhip_stats *st) {
     size_t need = (size_t)j->cw * j->ch * 3;
     if (out) {
         if (n < need) return fail(PCRHIP_EINVAL, "output buffer too small");
-        const double div = j->welford ? 1.0 : (double)j->d.spp;  // adaptive sums hold the means
-        for (size_t i = 0; i < need; i++) out[i] = (float)(j->sums[i] / div);
+        const uint32_t spp = j->welford ? 1u : (uint32_t)j->d.spp;  // adaptive sums hold the means
+        if ((spp & (spp - 1u)) == 0u) {
+            // x / 2^k and x * 2^-k are the same real number, rounded once: identical results
+            const double r = 1.0 / (double)spp;
+            for (size_t i = 0; i < need; i++) out[i] = (float)(j->sums[i] * r);
+        } else {
+            const double div = (double)spp;
+            for (size_t i = 0; i < need; i++) out[i] = (float)(j->sums[i] / div);
+        }
     }
     return job_stats(j, st);
 }
