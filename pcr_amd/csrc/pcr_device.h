@@ -196,6 +196,17 @@ __device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
 
 constexpr float kPiF = 3.14159274101257324f;
+// x / c for a constant c, with rc = RN(1/c): a product and one FMA-corrected
+// step.  For c = 5, 150 and kPiF this is the correctly rounded quotient for
+// +0 and for every x with 2^-100 <= |x| <= 2^100 (all 2^32 inputs checked by
+// tests/csrc/constdiv_check.c).  The caller guarantees that range.
+__device__ __forceinline__ float div_const_in_range(float x, float c, float rc) {
+    const float q0 = x * rc;
+    return __builtin_fmaf(__builtin_fmaf(-c, q0, x), rc, q0);
+}
+constexpr float kInvPiF = 0x1.45f306p-2f;   // RN(1 / kPiF)
+constexpr float kInv5F = 0x1.99999ap-3f;    // RN(1 / 5)
+constexpr float kInv150F = 0x1.b4e81cp-8f;  // RN(1 / 150)
 constexpr float kEps3 = 0.00100000004749745131f;  // (float)1e-3
 constexpr float kYbarIntegral = 106.895210266113281f;  // CIE::yBarIntegral() (CIE.h:79-88)
 
@@ -271,7 +282,8 @@ __device__ __forceinline__ V3 fm_cmf(float l) {
 template <class GA = GeoGlobal>
 __device__ __forceinline__ float spec_eval(const float *s, float l) {
     if (l < 400.f || l > 700.f) return 0.f;
-    float t = div_rn(fsub(l, 400.f), 5.f);
+    // l in [400, 700]: l - 400 is +0 or at least ulp(400) = 2^-15
+    float t = div_const_in_range(fsub(l, 400.f), 5.f, kInv5F);
     int i = (int)t;
     if (i >= 60) return GA::f1(s + 60);
     float f = fsub(t, (float)i);
@@ -281,7 +293,8 @@ __device__ __forceinline__ float sigmoid(float x) {
     return fadd(0.5f, div_rn(x, fmul(2.f, sqrt_rn(fadd(1.f, fmul(x, x))))));
 }
 __device__ __forceinline__ float eval_fit(float4 f, float l) {
-    float L = div_rn(fsub(l, 550.f), 150.f);
+    // a hero wavelength, l in [400, 700]: l - 550 is exact, +0 or at least 2^-15 in size
+    float L = div_const_in_range(fsub(l, 550.f), 150.f, kInv150F);
     float p = fadd(f.x, fmul(L, fadd(f.y, fmul(L, f.z))));
     return fmul(sigmoid(p), f.w);
 }

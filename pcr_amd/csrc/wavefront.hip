@@ -286,7 +286,18 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
         // (albedo/pi)*Le*G*A*w: albedo/pi per channel, and Le per channel for
         // the last light material seen
         float aop[4], le_c[4] = {0.f, 0.f, 0.f, 0.f};
-        _Pragma("unroll") for (int c = 0; c < 4; c++) aop[c] = div_rn(alb[c], kPiF);
+        // albedo / pi without a division where every channel is in the range the
+        // product-and-correction form rounds exactly (one test for the four)
+        bool aop_fast = true;
+        _Pragma("unroll") for (int c = 0; c < 4; c++) {
+            const float ab = fabsf(alb[c]);
+            aop_fast = aop_fast && ((ab >= 0x1p-100f && ab <= 0x1p100f) || __float_as_uint(alb[c]) == 0u);
+        }
+        if (__builtin_expect(aop_fast, 1)) {
+            _Pragma("unroll") for (int c = 0; c < 4; c++) aop[c] = div_const_in_range(alb[c], kPiF, kInvPiF);
+        } else {
+            _Pragma("unroll") for (int c = 0; c < 4; c++) aop[c] = div_rn(alb[c], kPiF);
+        }
         int le_mat = -1;
         for (int i = 0; i < J.shadow; i++) {
             int li = 0;
