@@ -115,7 +115,8 @@ struct pcrhip_job {
     int adam_t = 0;
     std::vector<float> guide_loss;  // mean training loss per pass
     uint64_t bytes = 0;
-    std::vector<double> sums;   // FP64 sums per pixel; adaptive: the Welford means
+    double *sums = nullptr;     // FP64 sums per pixel (pinned, from the context's pool); adaptive: the means
+    size_t sums_cap = 0;
     float4 *welford = nullptr;  // adaptive: device Welford state, after the sums in `accum`
     uint64_t counters[4] = {0, 0, 0, 0};
     Clock::time_point t_submit, t_admit, t_done;
@@ -144,8 +145,10 @@ struct pcrhip_ctx {
     uint64_t pool_bytes = 0;
     DJob *jobs_host = nullptr;  // pinned staging of the job table
     PassInfo *pass_host = nullptr;  // pinned staging of the pass sizes
-    double *sums_host = nullptr;    // pinned staging of a finished job's FP64 sums (worker thread only)
-    size_t sums_host_cap = 0;
+    // pinned result buffers of released jobs, by capacity in doubles: a finished job's sums are
+    // copied straight into one (a copy into pageable memory runs at a fraction of the link rate,
+    // and the GPU waits for it between frames)
+    std::multimap<size_t, double *> sums_pool;
     // captured pass graphs, keyed by launch shape; dropped when the queues move
     struct PassGraph {
         int nb, stack, block, grid;
@@ -812,9 +815,19 @@ void worker_main(pcrhip_ctx *c) {
         for (auto *j : run) {
             if (j->state == pcrhip_job::RUNNING && j->s_next >= j->d.spp) {
                 uint64_t n = (uint64_t)j->cw * j->ch * 3;
-                hipError_t e;
-                if (j->welford) {  // adaptive: the result is the Welford mean
-                    j->sums.resize(n);
+                hipError_t e = hipSuccess;
+                auto it = c->sums_pool.lower_bound(n);
+                if (it != c->sums_pool.end()) {
+                    j->sums = it->second;
+                    j->sums_cap = it->first;
+                    c->sums_pool.erase(it);
+                } else if ((e = hipHostMalloc(&j->sums, n * sizeof(double), 0)) == hipSuccess) {
+                    j->sums_cap = n;
+                } else {
+                    j->sums = nullptr;
+                }
+                if (e != hipSuccess) {
+                } else if (j->welford) {  // adaptive: the result is the Welford mean
                     std::vector<float4> w(2 * (size_t)j->cw * j->ch);
                     e = hipMemcpy(w.data(), j->welford, w.size() * sizeof(float4), hipMemcpyDeviceToHost);
                     for (size_t p = 0; p < n / 3; p++) {
@@ -823,21 +836,8 @@ void worker_main(pcrhip_ctx *c) {
                         j->sums[3 * p + 2] = w[2 * p].z;
                     }
                 } else {
-                    // through pinned staging: a copy into pageable memory runs at a fraction of the link
-                    // rate, and the GPU waits for it between frames
-                    e = hipSuccess;
-                    if (n > c->sums_host_cap) {
-                        (void)hipHostFree(c->sums_host);
-                        c->sums_host = nullptr;
-                        c->sums_host_cap = 0;
-                        e = hipHostMalloc(&c->sums_host, n * sizeof(double), 0);
-                        if (e == hipSuccess) c->sums_host_cap = n;
-                    }
-                    if (e == hipSuccess)
-                        e = hipMemcpyAsync(c->sums_host, j->accum, n * sizeof(double), hipMemcpyDeviceToHost,
-                                           c->stream);
+                    e = hipMemcpyAsync(j->sums, j->accum, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
                     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-                    if (e == hipSuccess) j->sums.assign(c->sums_host, c->sums_host + n);
                 }
                 finish_job(c, j, e == hipSuccess ? PCRHIP_OK : PCRHIP_EDEVICE,
                            e == hipSuccess ? "" : hipGetErrorString(e));
@@ -1181,7 +1181,8 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     (void)hipFree(c->ctrs);
     (void)hipFree(c->slot_counters);
     (void)hipHostFree(c->jobs_host);
-    (void)hipHostFree(c->sums_host);
+    for (auto &kv : c->sums_pool) (void)hipHostFree(kv.second);
+    c->sums_pool.clear();
     (void)hipStreamDestroy(c->stream);
     delete c;
     return PCRHIP_OK;
@@ -1397,7 +1398,7 @@ int pcrhip_job_result_sums(pcrhip_job *j, double *out, size_t n) {
     if (j->status != PCRHIP_OK) return fail(j->status, j->error);
     size_t need = (size_t)j->cw * j->ch * 3;
     if (n < need) return fail(PCRHIP_EINVAL, "output buffer too small");
-    std::memcpy(out, j->sums.data(), need * sizeof(double));
+    std::memcpy(out, j->sums, need * sizeof(double));
     return PCRHIP_OK;
 }
 
@@ -1411,6 +1412,11 @@ int pcrhip_job_release(pcrhip_job *j) {
             j->cancel.store(true);
             c->cv.notify_all();
             c->done_cv.wait(lk, [&] { return j->state == pcrhip_job::DONE; });
+        }
+        if (j->sums) {  // keep a few result buffers for the next jobs
+            if (c->sums_pool.size() < 4) c->sums_pool.emplace(j->sums_cap, j->sums);
+            else (void)hipHostFree(j->sums);
+            j->sums = nullptr;
         }
     }
     delete j;
