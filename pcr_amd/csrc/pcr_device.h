@@ -443,11 +443,25 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
             float bu = 0.f, bv = 0.f;
             int top = 0;  // entries below the current node
             int cur = S.root_ref;
-            for (;;) {
-                if (cur < 0) {
-                    int2 lf = G::i2(S.leaves + ~cur);
-                    for (int k = 0; k < lf.y; k++) tri_test<G>(S.tri, lf.x + k, o, d, closest, best, bu, bv);
-                } else {
+            // pop: drop entries whose entry distance is beyond the closest hit
+            auto pop = [&]() {
+                while (top > 0) {
+                    top--;
+                    float tn = st.tn[top * st.stride];
+                    if (tn > closest) continue;
+                    cur = st.idx[top * st.stride];
+                    return true;
+                }
+                return false;
+            };
+            // while-while: a wave walks inner nodes until each lane has reached a
+            // leaf (or ended), then tests those leaves together, instead of
+            // running node and leaf code in every turn; each lane's own order
+            // of nodes, leaves and pops is unchanged
+            bool live = true;
+            while (live) {
+                // inner nodes until this lane reaches a leaf or its walk ends
+                while (live && cur >= 0) {
                     const float4 *nd = S.bvh + 4 * cur;
                     float4 a = G::f4(nd), b = G::f4(nd + 1), c = G::f4(nd + 2), e = G::f4(nd + 3);
                     float tL, tR;
@@ -455,8 +469,6 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
                     bool hR = box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, closest, tR);
                     int rl = __float_as_int(e.x), rr = __float_as_int(e.y);
                     if (hL && hR) {
-                        // reference: `if (top + 2 > 64) continue;` with `top` counting
-                        // the entries below the current node
                         if (top + 2 <= 64 && top + 1 <= STACK) {
                             bool leftNear = tL <= tR;
                             st.idx[top * st.stride] = leftNear ? rr : rl;
@@ -470,18 +482,13 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
                     } else if (hR) {
                         if (top + 1 <= 64) { cur = rr; continue; }
                     }
+                    live = pop();
                 }
-                // pop: drop entries whose entry distance is beyond the closest hit
-                bool found = false;
-                while (top > 0) {
-                    top--;
-                    float tn = st.tn[top * st.stride];
-                    if (tn > closest) continue;
-                    cur = st.idx[top * st.stride];
-                    found = true;
-                    break;
+                if (live) {
+                    int2 lf = G::i2(S.leaves + ~cur);
+                    for (int k = 0; k < lf.y; k++) tri_test<G>(S.tri, lf.x + k, o, d, closest, best, bu, bv);
+                    live = pop();
                 }
-                if (!found) break;
             }
             if (best >= 0) {
                 h.kind = 2;
