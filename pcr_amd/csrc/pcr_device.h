@@ -355,21 +355,22 @@ __device__ __forceinline__ bool box_test(V3 o, V3 inv, float lx, float ly, float
     if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
     tmin = fmaxf(tmin, t1);
     tmax = fminf(tmax, t2);
-    if (tmin > tmax) return false;
     t1 = fmul(fsub(ly, o.y), inv.y);
     t2 = fmul(fsub(hy, o.y), inv.y);
     if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
     tmin = fmaxf(tmin, t1);
     tmax = fminf(tmax, t2);
-    if (tmin > tmax) return false;
     t1 = fmul(fsub(lz, o.z), inv.z);
     t2 = fmul(fsub(hz, o.z), inv.z);
     if (t1 > t2) { float s = t1; t1 = t2; t2 = s; }
     tmin = fmaxf(tmin, t1);
     tmax = fminf(tmax, t2);
-    if (tmin > tmax) return false;
+    // One test at the end instead of one per axis: tmin never decreases and
+    // tmax never increases (fmaxf/fminf skip a NaN operand), so once
+    // tmin > tmax it stays so, and the answer is that of testing after every
+    // axis; without the branches the wave runs straight-line code.
     tnear = tmin;
-    return true;
+    return !(tmin > tmax);
 }
 
 struct HitRec {
