@@ -321,28 +321,30 @@ __device__ __forceinline__ bool tri_test(const float4 *__restrict__ tri, int ti,
     V3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
-    if (fabsf(det) < 1e-6f) return false;
     V3 tv = sub(o, v0);
     float du = dot(tv, pv);
     // u = du * (1/det) with both roundings correct.  Decided without the
     // division where the outcome is certain: u < 0 exactly when du and det
     // have strictly opposite signs (du = +-0 gives u = +-0), and
     // |du| > |det| * 1.000001 forces u > 1 + 2^-23 after both roundings.
-    if ((du < 0.f && det > 0.f) || (du > 0.f && det < 0.f)) return false;
-    if (fabsf(du) > fmul(fabsf(det), 1.000001f)) return false;
+    // The rejections before the division are combined into one branch, and
+    // those after it into selects: the same answer as testing them in turn.
+    const bool pre = !(fabsf(det) < 1e-6f) && !((du < 0.f && det > 0.f) || (du > 0.f && det < 0.f)) &&
+                     !(fabsf(du) > fmul(fabsf(det), 1.000001f));
+    if (!pre) return false;
     float inv = div_rn(1.f, det);
     float u = fmul(du, inv);
-    if (u < 0.f || u > 1.f) return false;
     V3 qv = cross(tv, e1);
     float v = fmul(dot(d, qv), inv);
-    if (v < 0.f || fadd(u, v) > 1.f) return false;
     float t = fmul(dot(e2, qv), inv);
-    if (t <= 1e-6f || t >= closest) return false;
-    closest = t;
-    best = ti;
-    bu = u;
-    bv = v;
-    return true;
+    const bool hit = !(u < 0.f || u > 1.f) && !(v < 0.f || fadd(u, v) > 1.f) && !(t <= 1e-6f || t >= closest);
+    if (hit) {
+        closest = t;
+        best = ti;
+        bu = u;
+        bv = v;
+    }
+    return hit;
 }
 // Slab test against [lo, hi] for the segment (0, seg); inv = 1/d per axis.
 // tmin and tmax are never NaN, and fmaxf(a, NaN) = a = std::max(a, NaN),
