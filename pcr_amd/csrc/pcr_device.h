@@ -412,12 +412,11 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
         float td = sqrt_rn(fsub(r2, tSq));
         float t0 = fsub(rl, td);
         float t1 = fadd(rl, td);
-        if (t0 < 0) t0 = t1;
-        if (t0 < 0) continue;
-        if (t0 >= closest) continue;
-        closest = t0;
-        h.kind = 0;
-        h.prim = i;
+        t0 = t0 < 0 ? t1 : t0;
+        const bool take = !(t0 < 0) && !(t0 >= closest);
+        closest = take ? t0 : closest;
+        h.kind = take ? 0 : h.kind;
+        h.prim = take ? i : h.prim;
     }
     for (int i = 0; i < S.n_pl; i++) {
         float4 po = G::f4(S.pl + 4 * i), pu = G::f4(S.pl + 4 * i + 1), pv = G::f4(S.pl + 4 * i + 2),
@@ -471,19 +470,22 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
                     bool hL = box_test(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, closest, tL);
                     bool hR = box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, closest, tR);
                     int rl = __float_as_int(e.x), rr = __float_as_int(e.y);
-                    if (hL && hR) {
-                        if (top + 2 <= 64 && top + 1 <= STACK) {
-                            bool leftNear = tL <= tR;
-                            st.idx[top * st.stride] = leftNear ? rr : rl;
-                            st.tn[top * st.stride] = leftNear ? tR : tL;
-                            top++;
-                            cur = leftNear ? rl : rr;
-                            continue;
-                        }
-                    } else if (hL) {
-                        if (top + 1 <= 64) { cur = rl; continue; }
-                    } else if (hR) {
-                        if (top + 1 <= 64) { cur = rr; continue; }
+                    // the reference's branches as selects: push the far child and take the
+                    // near one, take the only child hit, or pop (also when a push or a
+                    // step would pass the reference's 64-entry stack)
+                    const bool both = hL && hR;
+                    const bool push = both && top + 2 <= 64 && top + 1 <= STACK;
+                    const bool one = !both && (hL || hR) && top + 1 <= 64;
+                    const bool leftNear = tL <= tR;
+                    if (push) {
+                        st.idx[top * st.stride] = leftNear ? rr : rl;
+                        st.tn[top * st.stride] = leftNear ? tR : tL;
+                    }
+                    top += push ? 1 : 0;
+                    const int next = push ? (leftNear ? rl : rr) : (hL ? rl : rr);
+                    if (push || one) {
+                        cur = next;
+                        continue;
                     }
                     live = pop();
                 }
