@@ -497,16 +497,19 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // runs once per wave.  Lineages keep the appends off a single counter: one
 // address took every wave's atomic, and its serialisation set the launch time
 // of small scenes.
-// Occupancy: 3 waves per SIMD (<= 168 VGPRs, a few dozen bytes of spills)
-// measured 20% faster than the unconstrained 2 waves at 185-195 VGPRs, and 4
-// waves (128 VGPRs, ~200 bytes of spills) slower.  Up to 24 stack entries,
-// 3 waves per SIMD also fit LDS (12 waves x 64 lanes x 24 x 8 bytes = 147 KB
-// of 160 KB); deeper stacks are limited by LDS and keep the compiler's choice.
+// Occupancy: 3 waves per SIMD (<= 168 VGPRs) measured 20% faster than the
+// unconstrained 2 waves at 185-195 VGPRs.  Small scenes read from LDS with an
+// 8-entry stack run at 4 waves per SIMD (128 VGPRs, ~120 bytes of spills per
+// lane; LDS at most 16 x 9 KB, 8 x 17 KB or 4 x 33 KB per CU): 10% faster than
+// 3.  Up to 24 stack entries, 3 waves per SIMD also fit LDS (12 waves x 64
+// lanes x 24 x 8 bytes = 147 KB of 160 KB); deeper stacks are limited by LDS
+// and keep the compiler's choice.
 // LDS_SCENE: every job of the pass shares one scene small enough for
 // geo_cap(BLOCK); each block copies it (geo_layout) into LDS first, and the
 // traversal, materials, lights and spectral tables are read there.
 template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 24 ? 3 : 1, STACK <= 24 ? 3 : 10)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
+    LDS_SCENE && STACK <= 8 ? 4 : STACK <= 24 ? 3 : 1, LDS_SCENE && STACK <= 8 ? 4 : STACK <= 24 ? 3 : 10)))
 void k_bounce(BounceArgs a, const DJob *__restrict__ jobs) {
     __shared__ int32_t s_idx[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
