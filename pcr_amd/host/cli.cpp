@@ -81,7 +81,8 @@ std::vector<Found> discover(const std::vector<std::string> &extra) {
                  "  --list-scenes           list discovered scenes and exit\n"
                  "  --mis --russian --stratified --aces --spectral\n"
                  "  --aa [--aa-samples N]   jittered primary rays (N x samples paths per pixel)\n"
-                 "  --adaptive --denoise --oidn   accepted; not provided by this backend\n"
+                 "  --adaptive              stop converged pixels (Welford over AA iterations of --samples)\n"
+                 "  --denoise --oidn        accepted; not provided by this backend\n"
                  "  --hero-samples N        hero wavelengths 1..4 (4)\n"
                  "  --seed N                NumGen seed (0: from the clock)\n"
                  "  --guided                learned directional proposal (trained during the render)\n"
