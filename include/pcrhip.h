@@ -283,8 +283,9 @@ int pcrhip_guide_forward(pcrhip_ctx *ctx, const float *params, const float *box,
  * weight_i, written to grad (param_count floats); loss receives the loss. */
 int pcrhip_guide_gradient(pcrhip_ctx *ctx, const float *params, const float *box, const float *recs, int n,
                           float *grad, float *loss);
-/* Mean training loss of each pass of a finished guided job (up to n values);
- * returns the number of passes through *count. */
+/* Mean training loss of each pass of a finished guided job that trained (up
+ * to n values): every pass but the last, whose network no later pass would
+ * use.  Returns the number of values through *count. */
 int pcrhip_job_guide_losses(pcrhip_job *job, float *out, int n, int *count);
 
 /* ------------------------------------------------------- measurement ----- */

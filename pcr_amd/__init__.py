@@ -378,7 +378,7 @@ class Job:
         return out, st
 
     def guide_losses(self):
-        """Mean training loss of each pass (guided jobs; after wait())."""
+        """Mean training loss of each pass that trained: all but the last (guided jobs; after wait())."""
         n = C.c_int()
         check(hip().pcrhip_job_guide_losses(self.h, None, 0, C.byref(n)))
         out = np.zeros(max(1, n.value), np.float32)

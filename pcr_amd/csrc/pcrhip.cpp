@@ -532,9 +532,9 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
     HIPCHK(pcr_launch_reset((uint32_t *)g.wsum, 2 * kGMaxSteps, c->stream));
     HIPCHK(pcr_launch_reset(g.n_rec, 1, c->stream));
     HIPCHK(pcr_launch_reset(g.n_part, kLineages * 32, c->stream));
-    // record about rec_cap of the pass's guided vertices (2 per path assumed)
+    // record about rec_cap of the pass's guided vertices (2 per path assumed); none when no training follows
     const double frac = std::min(1.0, (double)rec_cap / (2.0 * std::max<uint32_t>(n_gen, 1)));
-    const uint32_t keep = (uint32_t)std::min(16777216.0, std::ceil(frac * 16777216.0));
+    const uint32_t keep = S > 0 ? (uint32_t)std::min(16777216.0, std::ceil(frac * 16777216.0)) : 0u;
     for (int b = 0; b < sh.nb; b++) {
         BounceArgs a = bounce_args(c, sh, b);
         if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * b], c->stream));
@@ -550,6 +550,7 @@ int enqueue_guided_pass(pcrhip_ctx *c, const PassShape &sh, const std::vector<hi
     if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb], c->stream));
     HIPCHK(pcr_launch_resolve(&ra, sh.rgrid, c->stream));
     if (!ev.empty()) HIPCHK(hipEventRecord(ev[2 * sh.nb + 1], c->stream));
+    if (S == 0) return PCRHIP_OK;
     HIPCHK(pcr_launch_guide_pack(&g, c->grec, rec_cap, c->stream));
     HIPCHK(pcr_launch_guide_prep(&g, c->grec, rec_cap, c->pass_rad + segs[0].begin, S, c->stream));
     GuideTrain h{j->d.guide_lr > 0.f ? j->d.guide_lr : 5e-3f, 0.9f, 0.99f, 1e-8f, 0.f, 0.f};
@@ -693,7 +694,11 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     // launches (same kernels, same grid).
     const bool prof = c->profile.load();
     std::vector<hipEvent_t> ev;
-    const int S = guided ? (jobs[0]->d.guide_steps > 0 ? jobs[0]->d.guide_steps : kGDefaultSteps) : 0;
+    // Training steps after the pass.  A job's last pass trains nothing: no later pass would use the network (its
+    // image is the same either way), so it records no vertices either.
+    const int S = !guided || jobs[0]->s_next + jobs[0]->spp_pass >= jobs[0]->d.spp
+                      ? 0
+                      : (jobs[0]->d.guide_steps > 0 ? jobs[0]->d.guide_steps : kGDefaultSteps);
     if (guided) {
         if (prof && (rc = make_events(ev, 2 * nb + 2))) return rc;
         if ((rc = enqueue_guided_pass(c, sh, ev, (uint32_t)n_gen, segs, jobs[0], rec_cap, S))) return rc;
@@ -711,11 +716,11 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     HIPCHK(hipMemcpyAsync(sc.data(), c->slot_counters, sc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c->stream));
     float gl[2 * kGMaxSteps];
-    if (guided)
+    if (S > 0)
         HIPCHK(hipMemcpyAsync(gl, jobs[0]->net.wsum, 2 * S * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ctr[kCtrStride * nb]) return fail(PCRHIP_EDEVICE, "ray queue overflow");
-    if (guided) {
+    if (S > 0) {  // the mean training loss of each pass that trained
         float l = 0.f;
         int k = 0;
         for (int q = 0; q < S; q++)
