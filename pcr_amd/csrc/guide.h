@@ -38,7 +38,7 @@ constexpr int kGDefaultSteps = 128, kGDefaultRecords = 262144;
 // and keep cosine sampling, which spares their inference.
 constexpr int kGDefaultBounces = 3;
 // Largest guided pass (paths): training runs after every pass.
-constexpr uint64_t kGuidedPassPaths = 16ull << 20;
+constexpr uint64_t kGuidedPassPaths = 32ull << 20;
 
 // The inference kernels keep a wave's activations in registers: rays are the
 // N dimension of v_mfma_f32_16x16x32_bf16 (H^T = W^T X^T), so a layer's
