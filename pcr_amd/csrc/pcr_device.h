@@ -384,18 +384,28 @@ struct HitRec {
 
 // Per-thread traversal stack in LDS, entry-major so a wave's 64 lanes touch 64
 // consecutive dwords: idx[k * BLOCK + lane].
+// A traversal may keep only its first entries in LDS (scene_closest's LDS_N)
+// and the deeper ones in `deep`, entry-major over the launch's threads (the
+// thread's slot of entry k is deep[(k - LDS_N) * deep_stride + global thread
+// id]): a deep tree's full stack in LDS would cost a wave per SIMD, and
+// entries that deep are rarely reached.
 struct LdsStack {
     int32_t *idx;
     float *tn;
     int stride;
+    int2 *deep = nullptr;  // (node, entry distance bits)
+    int deep_stride = 0;   // threads of the launch
 };
+__device__ __forceinline__ int2 &deep_entry(const LdsStack &st, int k) {
+    return st.deep[(size_t)k * st.deep_stride + blockIdx.x * blockDim.x + threadIdx.x];
+}
 
 // Renderer::sceneIntersect semantics with the BVH traversal of Bvh::intersect.
 // The reference pushes both children (far first) and pops the near one; here
 // the near child is taken directly and only the far one is pushed, which is the
 // same visiting order.  Pushes that the reference's 64-entry stack would drop
 // are dropped here too (never reached for trees shallower than 63).
-template <int STACK, class G = GeoGlobal>
+template <int STACK, class G = GeoGlobal, int LDS_N = STACK>
 __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, float tmax, const LdsStack &st) {
     HitRec h;
     h.kind = -1;
@@ -449,9 +459,15 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
             auto pop = [&]() {
                 while (top > 0) {
                     top--;
-                    float tn = st.tn[top * st.stride];
-                    if (tn > closest) continue;
-                    cur = st.idx[top * st.stride];
+                    if (LDS_N >= STACK || top < LDS_N) {
+                        float tn = st.tn[top * st.stride];
+                        if (tn > closest) continue;
+                        cur = st.idx[top * st.stride];
+                    } else {
+                        const int2 e = deep_entry(st, top - LDS_N);
+                        if (__int_as_float(e.y) > closest) continue;
+                        cur = e.x;
+                    }
                     return true;
                 }
                 return false;
@@ -478,8 +494,12 @@ __device__ __forceinline__ HitRec scene_closest(const DScene &S, V3 o, V3 d, flo
                     const bool one = !both && (hL || hR) && top + 1 <= 64;
                     const bool leftNear = tL <= tR;
                     if (push) {
-                        st.idx[top * st.stride] = leftNear ? rr : rl;
-                        st.tn[top * st.stride] = leftNear ? tR : tL;
+                        if (LDS_N >= STACK || top < LDS_N) {
+                            st.idx[top * st.stride] = leftNear ? rr : rl;
+                            st.tn[top * st.stride] = leftNear ? tR : tL;
+                        } else {
+                            deep_entry(st, top - LDS_N) = make_int2(leftNear ? rr : rl, __float_as_int(leftNear ? tR : tL));
+                        }
                     }
                     top += push ? 1 : 0;
                     const int next = push ? (leftNear ? rl : rr) : (hL ? rl : rr);

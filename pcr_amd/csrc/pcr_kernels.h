@@ -137,6 +137,8 @@ struct BounceArgs {
                                   // and the lineages' work-stealing counters
     uint32_t *overflow;           // set if an append would exceed its region
     unsigned long long *counters; // paths, extension rays, shadow rays, emissive hits
+    int2 *deep;                   // traversal stack entries beyond the LDS part: bounce_deep_entries(stack)
+                                  // per thread of the launch, entry-major
 };
 
 struct ResolveArgs {
@@ -157,6 +159,10 @@ struct IntersectArgs {
 };
 
 }  // namespace pcr
+
+// Traversal stack entries of a global-memory scene's bounce kernel that live in
+// BounceArgs::deep rather than LDS: the 24-entry stack keeps 16 in LDS.
+__host__ __device__ constexpr int bounce_deep_entries(int stack) { return stack == 24 ? 8 : 0; }
 
 extern "C" {
 // lds_scene: every job of the pass uses one scene whose geo_layout fits

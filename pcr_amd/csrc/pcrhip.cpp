@@ -137,6 +137,8 @@ struct pcrhip_ctx {
     PassInfo *pass_dev = nullptr;
     QueueView qa{}, qb{};
     uint64_t qcap = 0;
+    int2 *deep = nullptr;  // traversal stack entries past the LDS part (24-entry stacks)
+    size_t deep_bytes = 0;
     float4 *pass_rad = nullptr;  // per-path channel radiance of the current pass
     uint64_t pass_rad_cap = 0;
     GuideRecord *grec = nullptr;  // training records of a guided pass
@@ -196,6 +198,20 @@ hipError_t alloc_queue(QueueView &q, uint64_t n) {
 void drop_graphs(pcrhip_ctx *c) {
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     c->graphs.clear();
+}
+
+// Room for the deep traversal-stack entries of a launch shape (captured
+// launches hold the address, so growing it drops them).
+int ensure_deep(pcrhip_ctx *c, int stack, int grid, int block) {
+    const size_t need = (size_t)bounce_deep_entries(stack) * (size_t)grid * (size_t)block * sizeof(int2);
+    if (need <= c->deep_bytes) return PCRHIP_OK;
+    drop_graphs(c);
+    (void)hipFree(c->deep);
+    c->deep = nullptr;
+    c->deep_bytes = 0;
+    HIPCHK(hipMalloc(&c->deep, need));
+    c->deep_bytes = need;
+    return PCRHIP_OK;
 }
 
 int ensure_queues(pcrhip_ctx *c, uint64_t need) {
@@ -473,6 +489,7 @@ BounceArgs bounce_args(pcrhip_ctx *c, const PassShape &sh, int b) {
     a.count_out = c->ctrs + kCtrStride * b;
     a.overflow = c->ctrs + kCtrStride * sh.nb;
     a.counters = c->slot_counters;
+    a.deep = c->deep;
     return a;
 }
 
@@ -689,6 +706,9 @@ int run_pass(pcrhip_ctx *c, std::vector<pcrhip_job *> &jobs, const pcrhip_config
     const PassShape sh{std::max(max_depth, 1), stack, cfg.block, c->n_cu * std::max(1, cfg.blocks_per_cu),
                        c->n_cu * 4, lds_scene};
     const int nb = sh.nb;
+    if (!sh.lds_scene && bounce_deep_entries(stack_class(stack)) > 0 &&
+        (rc = ensure_deep(c, stack_class(stack), sh.grid, pcr_bounce_block(stack, sh.block))))
+        return rc;
     // Per-kernel event timing needs the launches on the stream itself, so a
     // profiled pass of the graph schedule is issued as the same persistent
     // launches (same kernels, same grid).
@@ -1174,6 +1194,7 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     drop_graphs(c);
     free_queue(c->qa);
     free_queue(c->qb);
+    (void)hipFree(c->deep);
     (void)hipFree(c->pass_rad);
     (void)hipFree(c->grec);
     for (auto &kv : c->pool) (void)hipFree(kv.second);

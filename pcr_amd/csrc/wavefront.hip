@@ -164,7 +164,7 @@ struct ForkSink {
 // continues as a single ray; terminated paths (and forks, whose sub-paths go
 // to the sink) return false.  S is the job's scene; its traversal data are
 // read through GA (global memory, or the launch's LDS copy).
-template <int STACK, class GA>
+template <int STACK, class GA, int LDS_N>
 __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, PathState &ps, const LdsStack &st,
                                             BounceCounters &cnt, const ForkSink &fork) {
     if (ps.meta & ST_DEAD) {  // ended by the guiding kernel (direction below the surface)
@@ -189,7 +189,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
     }
 
     cnt.ext++;
-    HitRec h = scene_closest<STACK, GA>(S, ps.o, ps.d, 3.40282347e38f, st);
+    HitRec h = scene_closest<STACK, GA, LDS_N>(S, ps.o, ps.d, 3.40282347e38f, st);
     if (h.kind < 0) {
         write_radiance(J, ps);
         return false;
@@ -349,7 +349,7 @@ __device__ __forceinline__ bool bounce_path(const DJob &J, const DScene &S, Path
             // occlusion answer of the reference's unbounded search -- unchanged.
             float tb = fadd(fmul(sqrt_rn(d2), 1.001f), 1.f);
             if (!(tb < 3.40282347e38f)) tb = 3.40282347e38f;
-            HitRec sh = scene_closest<STACK, GA>(S, so, wi, tb, st);
+            HitRec sh = scene_closest<STACK, GA, LDS_N>(S, so, wi, tb, st);
             bool occl = false;
             if (sh.kind >= 0) {
                 // the occlusion rule needs the hit point (as hit_geometry
@@ -501,22 +501,31 @@ __device__ __forceinline__ void wave_flush(const LaneCounters &c, int slot, unsi
 // unconstrained 2 waves at 185-195 VGPRs.  Small scenes read from LDS with an
 // 8-entry stack run at 4 waves per SIMD (128 VGPRs, ~120 bytes of spills per
 // lane; LDS at most 16 x 9 KB, 8 x 17 KB or 4 x 33 KB per CU): 10% faster than
-// 3.  Up to 24 stack entries, 3 waves per SIMD also fit LDS (12 waves x 64
-// lanes x 24 x 8 bytes = 147 KB of 160 KB); deeper stacks are limited by LDS
-// and keep the compiler's choice.
+// 3.  The 24-entry stack keeps 16 entries in LDS and the rest in global memory,
+// which fits 4 waves as well (16 waves x 64 lanes x 16 x 8 bytes = 128 KB);
+// deeper stacks are limited by LDS and keep the compiler's choice.
 // LDS_SCENE: every job of the pass shares one scene small enough for
 // geo_cap(BLOCK); each block copies it (geo_layout) into LDS first, and the
 // traversal, materials, lights and spectral tables are read there.
+// Waves per SIMD of a variant (0: the compiler's choice): 4 for small LDS
+// scenes and for the 24-entry stack, whose entries past 16 live in global
+// memory (BounceArgs::deep) so that 4 waves fit LDS; 3 otherwise up to 24
+// entries.
+constexpr int bounce_waves(int stack, bool lds_scene) {
+    return (lds_scene && stack <= 8) || (!lds_scene && stack == 24) ? 4 : stack <= 24 ? 3 : 0;
+}
 template <bool FIRST, int STACK, int BLOCK, bool LDS_SCENE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(
-    LDS_SCENE && STACK <= 8 ? 4 : STACK <= 24 ? 3 : 1, LDS_SCENE && STACK <= 8 ? 4 : STACK <= 24 ? 3 : 10)))
+    bounce_waves(STACK, LDS_SCENE) ? bounce_waves(STACK, LDS_SCENE) : 1,
+    bounce_waves(STACK, LDS_SCENE) ? bounce_waves(STACK, LDS_SCENE) : 10)))
 void k_bounce(BounceArgs a, const DJob *__restrict__ jobs) {
-    __shared__ int32_t s_idx[STACK * BLOCK];
-    __shared__ float s_tn[STACK * BLOCK];
+    constexpr int LDS_N = LDS_SCENE ? STACK : STACK - bounce_deep_entries(STACK);
+    __shared__ int32_t s_idx[LDS_N * BLOCK];
+    __shared__ float s_tn[LDS_N * BLOCK];
     __shared__ GenSeg s_segs[kMaxSegs];
     __shared__ float4 s_geo[LDS_SCENE ? geo_cap(BLOCK) : 1];
     using GA = typename std::conditional<LDS_SCENE, GeoLds, GeoGlobal>::type;
-    LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK};
+    LdsStack st{s_idx + threadIdx.x, s_tn + threadIdx.x, BLOCK, a.deep, (int)(gridDim.x * BLOCK)};
     const int n_segs = a.pass->n_segs;
     const uint32_t lane = lane_id();
     GeoLayout gl{};
@@ -637,9 +646,9 @@ void k_bounce(BounceArgs a, const DJob *__restrict__ jobs) {
                     Sl.lights = (const DLight *)(s_geo + gl.light);
                     Sl.ltri = s_geo + gl.ltri;
                     Sl.tabs = (const float *)(s_geo + gl.tabs);
-                    alive = bounce_path<STACK, GA>(J, Sl, ps, st, cnt, fork);
+                    alive = bounce_path<STACK, GA, LDS_N>(J, Sl, ps, st, cnt, fork);
                 } else {
-                    alive = bounce_path<STACK, GA>(J, J.S, ps, st, cnt, fork);
+                    alive = bounce_path<STACK, GA, LDS_N>(J, J.S, ps, st, cnt, fork);
                 }
                 lc.paths += FIRST ? 1u : 0u;
                 lc.ext += cnt.ext;
