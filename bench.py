@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # were measured with (the queue records a launch reads and writes depend on the
 # paths per pass, not on the block shape):
 # profiles/k_bounce_cornell-c1_720x720_d8_64spp_hbm_traffic.md.
-TRAFFIC_BYTES_PER_LAUNCH = 9539654267
+TRAFFIC_BYTES_PER_LAUNCH = 9541160182
 TRAFFIC_CONFIG = dict(target_pass_paths=64 << 20)
 SCHEDULES = {0: "graph", 1: "persistent", 2: "staged"}
 
