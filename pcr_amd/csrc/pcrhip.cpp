@@ -116,7 +116,7 @@ struct pcrhip_job {
     std::vector<float> guide_loss;  // mean training loss per pass
     uint64_t bytes = 0;
     double *sums = nullptr;     // FP64 sums per pixel (pinned, from the context's pool); adaptive: the means
-    size_t sums_cap = 0;
+    size_t sums_cap = 0;  // bytes (a size class)
     float4 *welford = nullptr;  // adaptive: device Welford state, after the sums in `accum`
     uint64_t counters[4] = {0, 0, 0, 0};
     Clock::time_point t_submit, t_admit, t_done;
@@ -147,10 +147,12 @@ struct pcrhip_ctx {
     uint64_t pool_bytes = 0;
     DJob *jobs_host = nullptr;  // pinned staging of the job table
     PassInfo *pass_host = nullptr;  // pinned staging of the pass sizes
-    // pinned result buffers of released jobs, by capacity in doubles: a finished job's sums are
+    // pinned result buffers of released jobs, by size class in bytes: a finished job's sums are
     // copied straight into one (a copy into pageable memory runs at a fraction of the link rate,
-    // and the GPU waits for it between frames)
+    // and the GPU waits for it between frames; hipHostMalloc and hipHostFree cost far more than
+    // a small job's copy)
     std::multimap<size_t, double *> sums_pool;
+    uint64_t sums_pool_bytes = 0;
     // captured pass graphs, keyed by launch shape; dropped when the queues move
     struct PassGraph {
         int nb, stack, block, grid;
@@ -277,6 +279,7 @@ void fill_djob(const pcrhip_job *j, DJob &J) {
 }
 
 constexpr uint64_t kPoolKeepBytes = 8ull << 30;  // freed accumulators kept for reuse
+constexpr uint64_t kSumsKeepBytes = 512ull << 20;  // pinned result buffers kept for reuse
 
 uint64_t size_class(uint64_t bytes) {
     uint64_t c = 64 << 10;
@@ -293,6 +296,17 @@ hipError_t pool_get(pcrhip_ctx *c, uint64_t cls, void **p) {
         return hipSuccess;
     }
     return hipMalloc(p, cls);
+}
+// Pinned result buffer of size class cls (bytes); caller holds c->m.
+hipError_t sums_get(pcrhip_ctx *c, uint64_t cls, double **p) {
+    auto it = c->sums_pool.find(cls);
+    if (it != c->sums_pool.end()) {
+        *p = it->second;
+        c->sums_pool.erase(it);
+        c->sums_pool_bytes -= cls;
+        return hipSuccess;
+    }
+    return hipHostMalloc((void **)p, cls, 0);
 }
 void pool_put(pcrhip_ctx *c, void *p, uint64_t cls) {
     if (!p) return;
@@ -837,33 +851,36 @@ void worker_main(pcrhip_ctx *c) {
                 if (j->state == pcrhip_job::RUNNING) finish_job(c, j, rc, err);
             continue;
         }
+        // Finished jobs: every result copy is queued, then one wait covers them all.
+        std::vector<std::pair<pcrhip_job *, hipError_t>> fin;
         for (auto *j : run) {
-            if (j->state == pcrhip_job::RUNNING && j->s_next >= j->d.spp) {
-                uint64_t n = (uint64_t)j->cw * j->ch * 3;
-                hipError_t e = hipSuccess;
-                auto it = c->sums_pool.lower_bound(n);
-                if (it != c->sums_pool.end()) {
-                    j->sums = it->second;
-                    j->sums_cap = it->first;
-                    c->sums_pool.erase(it);
-                } else if ((e = hipHostMalloc(&j->sums, n * sizeof(double), 0)) == hipSuccess) {
-                    j->sums_cap = n;
-                } else {
-                    j->sums = nullptr;
+            if (j->state != pcrhip_job::RUNNING || j->s_next < j->d.spp) continue;
+            const uint64_t n = (uint64_t)j->cw * j->ch * 3;
+            const uint64_t cls = size_class(n * sizeof(double));
+            hipError_t e = sums_get(c, cls, &j->sums);
+            if (e == hipSuccess) {
+                j->sums_cap = cls;
+            } else {
+                j->sums = nullptr;
+            }
+            if (e != hipSuccess) {
+            } else if (j->welford) {  // adaptive: the result is the Welford mean
+                std::vector<float4> w(2 * (size_t)j->cw * j->ch);
+                e = hipMemcpy(w.data(), j->welford, w.size() * sizeof(float4), hipMemcpyDeviceToHost);
+                for (size_t p = 0; p < n / 3; p++) {
+                    j->sums[3 * p] = w[2 * p].x;
+                    j->sums[3 * p + 1] = w[2 * p].y;
+                    j->sums[3 * p + 2] = w[2 * p].z;
                 }
-                if (e != hipSuccess) {
-                } else if (j->welford) {  // adaptive: the result is the Welford mean
-                    std::vector<float4> w(2 * (size_t)j->cw * j->ch);
-                    e = hipMemcpy(w.data(), j->welford, w.size() * sizeof(float4), hipMemcpyDeviceToHost);
-                    for (size_t p = 0; p < n / 3; p++) {
-                        j->sums[3 * p] = w[2 * p].x;
-                        j->sums[3 * p + 1] = w[2 * p].y;
-                        j->sums[3 * p + 2] = w[2 * p].z;
-                    }
-                } else {
-                    e = hipMemcpyAsync(j->sums, j->accum, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-                    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-                }
+            } else {
+                e = hipMemcpyAsync(j->sums, j->accum, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+            }
+            fin.emplace_back(j, e);
+        }
+        if (!fin.empty()) {
+            const hipError_t es = hipStreamSynchronize(c->stream);
+            for (auto &[j, e] : fin) {
+                if (e == hipSuccess) e = es;
                 finish_job(c, j, e == hipSuccess ? PCRHIP_OK : PCRHIP_EDEVICE,
                            e == hipSuccess ? "" : hipGetErrorString(e));
             }
@@ -1209,6 +1226,7 @@ int pcrhip_destroy(pcrhip_ctx *c) {
     (void)hipHostFree(c->jobs_host);
     for (auto &kv : c->sums_pool) (void)hipHostFree(kv.second);
     c->sums_pool.clear();
+    c->sums_pool_bytes = 0;
     (void)hipStreamDestroy(c->stream);
     delete c;
     return PCRHIP_OK;
@@ -1439,12 +1457,13 @@ int pcrhip_job_release(pcrhip_job *j) {
             c->cv.notify_all();
             c->done_cv.wait(lk, [&] { return j->state == pcrhip_job::DONE; });
         }
-        if (j->sums) {  // keep a few result buffers for the next jobs
-            if (c->sums_pool.size() < 4) c->sums_pool.emplace(j->sums_cap, j->sums);
-            else (void)hipHostFree(j->sums);
+        if (j->sums && c->sums_pool_bytes + j->sums_cap <= kSumsKeepBytes) {  // kept for the next jobs
+            c->sums_pool.emplace(j->sums_cap, j->sums);
+            c->sums_pool_bytes += j->sums_cap;
             j->sums = nullptr;
         }
     }
+    if (j->sums) (void)hipHostFree(j->sums);  // outside the lock: it can wait for the device
     delete j;
     return PCRHIP_OK;
 }

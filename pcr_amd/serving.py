@@ -130,7 +130,7 @@ def summarize(recs, makespan, isolated=None):
 
 
 def benchmark(stream, scene_dir=os.path.join(HERE, "scenes"), device=0, keep_sums=False, progress=10.0, log=print,
-              admission=ADMIT_SHORTEST):
+              admission=ADMIT_SHORTEST, slots=0):
     """Runs the stream batched, then one at a time.  Returns (report, batched
     records, one-at-a-time records)."""
     ctx = Context(device)
@@ -138,7 +138,7 @@ def benchmark(stream, scene_dir=os.path.join(HERE, "scenes"), device=0, keep_sum
         dscenes = [ctx.upload(HostScene(os.path.join(scene_dir, s + ".json"))) for s in SCENES]
         cfg0 = ctx.config()
         log("batched: %d jobs" % len(stream))
-        ctx.set_config(admission=admission)
+        ctx.set_config(admission=admission, max_active_jobs=slots or cfg0.max_active_jobs)
         rb, mb = run_stream(ctx, dscenes, stream, keep_sums, progress)
         log("one at a time: %d jobs" % len(stream))
         ctx.set_config(max_active_jobs=1, admission=ADMIT_FIFO)
@@ -168,16 +168,17 @@ def main(argv=None):
     ap.add_argument("--spp-max", type=int, default=1024)
     ap.add_argument("--admission", choices=("shortest", "fifo"), default="shortest")
     ap.add_argument("--res", type=str, default=",".join(map(str, RESOLUTIONS)))
+    ap.add_argument("--slots", type=int, default=0, help="job slots of the batched run (default: the library's)")
     ap.add_argument("--check", action="store_true", help="also check every job's result is identical in both runs")
     a = ap.parse_args(argv)
     stream = make_stream(a.jobs, a.rate, a.seed, tuple(int(x) for x in a.res.split(",")), (a.spp_min, a.spp_max))
     report, rb, ro = benchmark(stream, keep_sums=a.check,
-                               admission=ADMIT_SHORTEST if a.admission == "shortest" else ADMIT_FIFO)
+                               admission=ADMIT_SHORTEST if a.admission == "shortest" else ADMIT_FIFO, slots=a.slots)
     if a.check:
         same = all(np.array_equal(x["sums"].view(np.uint64), y["sums"].view(np.uint64)) for x, y in zip(rb, ro))
         report["alone_equals_mixed"] = same
     report["stream"] = dict(seed=a.seed, rate=a.rate, spp=[a.spp_min, a.spp_max], res=a.res, depth=[2, 8],
-                            scenes=SCENES, admission=a.admission,
+                            scenes=SCENES, admission=a.admission, slots=a.slots or None,
                             offered_mpaths=round(sum(s["desc"]["width"] ** 2 * s["desc"]["spp"] for s in stream) / 1e6,
                                                  1))
     print(json.dumps(report), flush=True)
