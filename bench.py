@@ -11,7 +11,10 @@ the device during timing.  Prints one JSON line.
 import argparse
 import json
 import os
+import re
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -46,6 +49,8 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_BYTES_PER_LAUNCH = 9541160182
 TRAFFIC_CONFIG = dict(target_pass_paths=64 << 20)
 SCHEDULES = {0: "graph", 1: "persistent", 2: "staged"}
+# The reference CLI, compiled from the reference's own sources by oracle/ref.mk.
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "pcr-ref-cli")
 
 
 def parse():
@@ -98,7 +103,7 @@ def copy_gbs(mib=1024, reps=5):
     return gbs
 
 
-def cpu_baseline(scene_path, target_s):
+def cpu_baseline(scene_path, target_s):  # the oracle port
     """The CPU oracle's non-branching estimator on a bounded sample of the same
     frame: the full frame at the first k samples per pixel, k chosen
     from a 1-sample calibration run so the sample takes about target_s; all
@@ -121,6 +126,48 @@ def cpu_baseline(scene_path, target_s):
             "sample": "oracle non-branching estimator: the %dx%d frame at samples 0..%d of %d, depth %d, "
                       "%d shadow rays, spectral; %d paths in %.1f s on %d threads"
                       % (W, H, k - 1, spp, d, S, paths, dt, threads)}
+
+
+def cpu_baseline_reference(target_s):
+    """The reference CLI's CPU renderer on the same scene and settings (depth,
+    shadow rays, spectral with 4 hero wavelengths), one indirect sample per hit
+    (code/Renderer.cpp:468-889) and k jittered primary rays per pixel
+    (--aa --aa-samples k), k chosen from a 1-ray calibration run so the sample
+    takes about target_s.  The CLI runs one thread per hardware thread
+    (code/Renderer.cpp:158-161); its CPU affinity is limited to 16 cores, the
+    host's share.  Time is the CLI's own "Render took N ms" clock (scene load
+    and PNG write excluded)."""
+    W, H, d, S = WORKLOAD["width"], WORKLOAD["height"], WORKLOAD["depth"], WORKLOAD["shadow"]
+    scenes = os.path.join(ROOT, "pcr_amd", "scenes")
+    cores = sorted(os.sched_getaffinity(0))[:16]
+
+    def run(k):
+        with tempfile.TemporaryDirectory() as od:
+            args = [REF_CLI, "--scene", WORKLOAD["scene"], "--scenes-dir", scenes, "-d", str(d), "-s", "1",
+                    "-S", str(S), "-w", str(W), "--height", str(H), "--spectral", "--hero-samples",
+                    str(WORKLOAD["hero"]), "-o", od]
+            if k > 1:
+                args += ["--aa", "--aa-samples", str(k)]
+            keep = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, cores)  # inherited by the child
+            try:
+                p = subprocess.run(args, cwd=scenes, capture_output=True, text=True, timeout=600)
+            finally:
+                os.sched_setaffinity(0, keep)
+        m = re.search(r"Render took (\d+) ms", p.stdout + p.stderr)
+        if p.returncode != 0 or not m:
+            raise RuntimeError("reference CLI failed (%d): %s" % (p.returncode, (p.stdout + p.stderr)[-400:]))
+        return int(m.group(1)) * 1e-3
+
+    t1 = run(1)
+    k = max(2, min(256, int(round(target_s / max(t1, 1e-3)))))
+    dt = run(k)
+    paths = W * H * k
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": len(cores), "kind": "reference",
+            "sample": "reference CLI (oracle/_ref/pcr-ref-cli) CPU renderer: the %dx%d %s frame, %d jittered "
+                      "primary rays per pixel, 1 indirect sample per hit, depth %d, %d shadow rays, spectral; "
+                      "%d paths in %.1f s (its own render clock), affinity %d cores"
+                      % (W, H, WORKLOAD["scene"], k, d, S, paths, dt, len(cores))}
 
 
 def main():
@@ -219,7 +266,12 @@ def main():
     ds.release()
     ctx.close()
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scene_path, a.cpu_seconds)
+        port = cpu_baseline(scene_path, a.cpu_seconds)
+        if os.path.exists(REF_CLI):
+            out["cpu_baseline"] = cpu_baseline_reference(a.cpu_seconds)
+            out["cpu_port"] = port
+        else:
+            out["cpu_baseline"] = port
     print(json.dumps(out), flush=True)
 
 
