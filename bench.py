@@ -267,8 +267,14 @@ def main():
     ctx.close()
     if not a.no_cpu_baseline:
         port = cpu_baseline(scene_path, a.cpu_seconds)
+        ref = None
         if os.path.exists(REF_CLI):
-            out["cpu_baseline"] = cpu_baseline_reference(a.cpu_seconds)
+            try:
+                ref = cpu_baseline_reference(a.cpu_seconds)
+            except RuntimeError as e:  # e.g. a scene primitive the reference's loader does not know
+                port["note"] = "reference CLI not used: %s" % str(e).splitlines()[0][:200]
+        if ref:
+            out["cpu_baseline"] = ref
             out["cpu_port"] = port
         else:
             out["cpu_baseline"] = port
