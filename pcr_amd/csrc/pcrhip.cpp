@@ -102,6 +102,7 @@ struct pcrhip_job {
     int status = PCRHIP_OK;
     std::string error;
     std::atomic<bool> cancel{false};
+    std::atomic<bool> done{false};  // state == DONE, readable without the context lock
     int slot = -1;
     int s_next = 0;        // first sample of the next pass
     int spp_pass = 1;      // samples in the current pass
@@ -323,6 +324,7 @@ void finish_job(pcrhip_ctx *c, pcrhip_job *j, int status, const std::string &err
     j->status = status;
     j->error = err;
     j->state = pcrhip_job::DONE;
+    j->done.store(true, std::memory_order_release);
     j->t_done = Clock::now();
     if (j->slot >= 0) c->active[j->slot] = nullptr;
     j->slot = -1;
@@ -1392,6 +1394,8 @@ int pcrhip_job_cancel(pcrhip_job *j) {
 int pcrhip_job_wait(pcrhip_job *j, int timeout_ms) {
     if (!j) return fail(PCRHIP_EINVAL, "null job");
     pcrhip_ctx *c = j->ctx;
+    // a poll of an unfinished job does not contend for the lock the serving thread takes between passes
+    if (timeout_ms == 0 && !j->done.load(std::memory_order_acquire)) return fail(PCRHIP_ETIMEOUT, "job still running");
     std::unique_lock<std::mutex> lk(c->m);
     auto pred = [&] { return j->state == pcrhip_job::DONE; };
     if (timeout_ms < 0) c->done_cv.wait(lk, pred);
