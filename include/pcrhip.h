@@ -224,7 +224,9 @@ int pcrhip_job_progress(pcrhip_job *job, int *rows_done, int *rows_total);
 /* Like the cancelRequested hook: the job stops at the next pass boundary and
  * its result reports PCRHIP_ECANCELLED. */
 int pcrhip_job_cancel(pcrhip_job *job);
-/* Waits up to timeout_ms (< 0: forever); PCRHIP_ETIMEOUT if still running. */
+/* Waits up to timeout_ms (< 0: forever); PCRHIP_ETIMEOUT if still running.
+ * timeout_ms == 0 polls: an unfinished job returns at once, without taking
+ * the context lock that the serving thread uses between passes. */
 int pcrhip_job_wait(pcrhip_job *job, int timeout_ms);
 
 typedef struct {
